@@ -1,0 +1,183 @@
+"""Periodic hexahedral box meshes in FLEXI's connectivity format.
+
+This is host-side set-up: it produces exactly the arrays FLEXI's mesh reader and
+``setLocalSideIDs``/``fillMeshInfo`` hand to the DG operator (src/mesh/prepare_mesh.f90:50-737,
+src/mesh/mesh_vars.f90:85-94):
+
+* ``NodeCoords(3,0:NGeo,0:NGeo,0:NGeo,nElems)`` on equidistant nodes (HOPR convention,
+  NodeTypeVISU, src/mesh/metrics.f90:75-156),
+* ``ElemToSide(2,6,nElems)`` = (SideID, flip) per local side,
+* ``SideToElem(5,nSides)`` = (ElemID, nbElemID, locSide, nbLocSide, flip),
+
+all 1-based like the Fortran arrays. A periodic box has no boundary sides, so every side is an
+inner side (``firstInnerSide = 1``, ``lastInnerSide = nSides``; src/mesh/mesh.f90:257-268).
+
+Local side ids follow ``flexi.h``: 1 zeta-, 2 eta-, 3 xi+, 4 eta+, 5 xi-, 6 zeta+.
+The master of a side is the element that reaches it first in element order (flip 0); the
+slave's flip is found geometrically by matching the face corners through the side-to-volume
+mapping of src/mesh/mappings.f90:188-446, so it is right for any orientation.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+XI_MINUS, XI_PLUS, ETA_MINUS, ETA_PLUS, ZETA_MINUS, ZETA_PLUS = 5, 3, 2, 4, 1, 6
+
+# Direction (0,1,2) and sign (-1,+1) of each local side.
+_SIDE_DIR = {ZETA_MINUS: (2, -1), ETA_MINUS: (1, -1), XI_PLUS: (0, +1),
+             ETA_PLUS: (1, +1), XI_MINUS: (0, -1), ZETA_PLUS: (2, +1)}
+
+
+def flip_s2m(N: int, p: int, q: int, flip: int) -> tuple[int, int]:
+    """``Flip_S2M`` (src/mesh/mappings.f90:188-221), 3D branch."""
+    if flip == 0:
+        return p, q
+    if flip == 1:
+        return q, p
+    if flip == 2:
+        return N - p, q
+    if flip == 3:
+        return N - q, N - p
+    if flip == 4:
+        return p, N - q
+    raise ValueError(flip)
+
+
+def cgns_side_to_vol(N: int, l: int, p: int, q: int, s: int) -> tuple[int, int, int]:
+    """``CGNS_SideToVol`` (src/mesh/mappings.f90:274-309), 3D branch."""
+    return {XI_MINUS: (l, q, p), XI_PLUS: (N - l, p, q), ETA_MINUS: (p, l, q),
+            ETA_PLUS: (N - p, N - l, q), ZETA_MINUS: (q, p, l), ZETA_PLUS: (p, q, N - l)}[s]
+
+
+def cgns_side_to_vol2(N: int, p: int, q: int, s: int) -> tuple[int, int]:
+    """``CGNS_SideToVol2`` (src/mesh/mappings.f90:315-346), 3D branch."""
+    return {XI_MINUS: (q, p), XI_PLUS: (p, q), ETA_MINUS: (p, q),
+            ETA_PLUS: (N - p, q), ZETA_MINUS: (q, p), ZETA_PLUS: (p, q)}[s]
+
+
+def side_to_vol(N: int, l: int, p: int, q: int, flip: int, s: int) -> tuple[int, int, int]:
+    """``SideToVol`` (src/mesh/mappings.f90:421-431)."""
+    a, b = flip_s2m(N, p, q, flip)
+    return cgns_side_to_vol(N, l, a, b, s)
+
+
+def side_to_vol2(N: int, p: int, q: int, flip: int, s: int) -> tuple[int, int]:
+    """``SideToVol2`` (src/mesh/mappings.f90:436-446)."""
+    a, b = flip_s2m(N, p, q, flip)
+    return cgns_side_to_vol2(N, a, b, s)
+
+
+def build_s2v2(N: int) -> np.ndarray:
+    """``S2V2(2,0:N,0:N,0:4,1:6)`` as a Fortran-ordered int32 array."""
+    out = np.zeros((2, N + 1, N + 1, 5, 6), dtype=np.int32, order="F")
+    for s in range(1, 7):
+        for f in range(5):
+            for q in range(N + 1):
+                for p in range(N + 1):
+                    out[:, p, q, f, s - 1] = side_to_vol2(N, p, q, f, s)
+    return out
+
+
+@dataclass
+class Mesh:
+    """A conforming hexahedral mesh in FLEXI's array layout (all Fortran-ordered)."""
+
+    nElems: int
+    nSides: int
+    NGeo: int
+    NodeCoords: np.ndarray   # (3,NGeo+1,NGeo+1,NGeo+1,nElems) float64
+    ElemToSide: np.ndarray   # (2,6,nElems) int32, 1-based SideID
+    SideToElem: np.ndarray   # (5,nSides) int32, 1-based ElemIDs
+    shape: tuple[int, int, int]
+    box_lo: tuple[float, float, float]
+    box_len: tuple[float, float, float]
+
+    @property
+    def firstInnerSide(self) -> int:
+        return 1
+
+    @property
+    def lastInnerSide(self) -> int:
+        return self.nSides
+
+
+def periodic_box(nx: int, ny: int, nz: int,
+                 lo=(-np.pi, -np.pi, -np.pi), length=(2 * np.pi, 2 * np.pi, 2 * np.pi),
+                 NGeo: int = 1,
+                 deform: Optional[Callable[[np.ndarray], np.ndarray]] = None) -> Mesh:
+    """Fully periodic box of ``nx*ny*nz`` hexahedra (element id = ix + nx*(iy + ny*iz) + 1).
+
+    ``deform`` maps an array of points ``(..., 3)`` to displaced points; it must be periodic
+    on the box so that the periodic faces still match (e.g. ``x + 0.1*sin x sin y sin z`` on
+    ``[-pi,pi]^3``). Each direction needs at least two elements so that the corner matching
+    that determines the flips is unambiguous.
+    """
+    if min(nx, ny, nz) < 2:
+        raise ValueError("periodic_box needs at least 2 elements per direction")
+    K = nx * ny * nz
+    dims = (nx, ny, nz)
+    h = [length[d] / dims[d] for d in range(3)]
+
+    # Equidistant geometry nodes of every element.
+    xi = np.linspace(0.0, 1.0, NGeo + 1)
+    NodeCoords = np.zeros((3, NGeo + 1, NGeo + 1, NGeo + 1, K), order="F")
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    elem_ijk = np.stack([ix.ravel(order="F"), iy.ravel(order="F"), iz.ravel(order="F")], axis=1)
+    a, b, c = np.meshgrid(xi, xi, xi, indexing="ij")
+    for e in range(K):
+        ex, ey, ez = elem_ijk[e]
+        pts = np.stack([lo[0] + (ex + a) * h[0], lo[1] + (ey + b) * h[1], lo[2] + (ez + c) * h[2]], axis=-1)
+        if deform is not None:
+            pts = deform(pts)
+        NodeCoords[:, :, :, :, e] = np.moveaxis(pts, -1, 0)
+
+    def elem_id(ex, ey, ez):
+        return (ex % nx) + nx * ((ey % ny) + ny * (ez % nz))
+
+    def corner_lattice(e, ls, p, q, flip):
+        i, j, k = side_to_vol(1, 0, p, q, flip, ls)
+        ex, ey, ez = elem_ijk[e]
+        return ((ex + i) % nx, (ey + j) % ny, (ez + k) % nz)
+
+    ElemToSide = np.zeros((2, 6, K), dtype=np.int32, order="F")
+    s2e = []
+    assigned = {}
+    for e in range(K):
+        ex, ey, ez = elem_ijk[e]
+        for ls in range(1, 7):
+            if (e, ls) in assigned:
+                continue
+            d, sgn = _SIDE_DIR[ls]
+            off = [0, 0, 0]
+            off[d] = sgn
+            nb = elem_id(ex + off[0], ey + off[1], ez + off[2])
+            nbls = [s for s, (dd, ss) in _SIDE_DIR.items() if dd == d and ss == -sgn][0]
+            master = [corner_lattice(e, ls, p, q, 0) for p in (0, 1) for q in (0, 1)]
+            flip = None
+            for f in range(5):
+                if [corner_lattice(nb, nbls, p, q, f) for p in (0, 1) for q in (0, 1)] == master:
+                    flip = f
+                    break
+            if flip is None or flip == 0:
+                raise RuntimeError("no consistent slave orientation found")
+            sid = len(s2e) + 1
+            s2e.append((e + 1, nb + 1, ls, nbls, flip))
+            assigned[(e, ls)] = (sid, 0)
+            assigned[(nb, nbls)] = (sid, flip)
+    for (e, ls), (sid, flip) in assigned.items():
+        ElemToSide[0, ls - 1, e] = sid
+        ElemToSide[1, ls - 1, e] = flip
+    SideToElem = np.asfortranarray(np.array(s2e, dtype=np.int32).T)
+    return Mesh(nElems=K, nSides=len(s2e), NGeo=NGeo, NodeCoords=NodeCoords, ElemToSide=ElemToSide,
+                SideToElem=SideToElem, shape=dims, box_lo=tuple(lo), box_len=tuple(length))
+
+
+def sine_deform(amplitude: float = 0.1) -> Callable[[np.ndarray], np.ndarray]:
+    """``x_i <- x_i + a*sin(x)sin(y)sin(z)`` applied to every coordinate (2*pi-periodic)."""
+    def f(pts: np.ndarray) -> np.ndarray:
+        s = amplitude * np.sin(pts[..., 0]) * np.sin(pts[..., 1]) * np.sin(pts[..., 2])
+        return pts + s[..., None]
+    return f

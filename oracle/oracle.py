@@ -1,0 +1,237 @@
+"""ctypes loader for the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` may import
+this module, and only to check results or to time the CPU baseline. The product path
+(``flexi_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+NODE_G, NODE_GL, NODE_CL, NODE_VISU = 1, 2, 3, 4
+RIEMANN_LF, RIEMANN_ROE, RIEMANN_ROEENTROPYFIX = 1, 3, 33
+
+
+class Params(C.Structure):
+    _fields_ = [("N", C.c_int), ("nodeType", C.c_int), ("splitDG", C.c_int), ("riemann", C.c_int),
+                ("kappa", C.c_double), ("R", C.c_double), ("Pr", C.c_double), ("mu0", C.c_double)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _setup(_lib)
+    return _lib
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+
+
+def _setup(L):
+    P = C.POINTER(Params)
+    v = C.c_void_p
+    L.or_nodes_weights.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+    L.or_derivative_matrix.argtypes = [C.c_int, C.c_int, _dp]
+    L.or_vandermonde.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+    L.or_dg_basis.argtypes = [C.c_int, C.c_int] + [_dp] * 12
+    L.or_mappings.argtypes = [C.c_int, _ip, _ip, _ip]
+    L.or_calc_metrics.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _ip] + [_dp] * 10
+    L.or_cons_to_prim.argtypes = [P, _dp, _dp]
+    L.or_prim_to_cons.argtypes = [P, _dp, _dp]
+    L.or_riemann_point.argtypes = [P] + [_dp] * 8
+    L.or_split_volume_flux_pi.argtypes = [_dp] * 7
+    L.or_eval_diff_flux_point.argtypes = [P] + [_dp] * 7
+    L.or_init.restype = v
+    L.or_init.argtypes = [P, C.c_int, C.c_int, _ip, _ip] + [_dp] * 8
+    L.or_free.argtypes = [v]
+    L.or_set_threads.argtypes = [C.c_int]
+    L.or_dg_time_derivative.argtypes = [v, C.c_double, _dp, _dp]
+    L.or_get_gradients.argtypes = [v, _dp, _dp, _dp]
+    L.or_get_faces.argtypes = [v, _dp, _dp, _dp]
+    L.or_calc_timestep.restype = C.c_double
+    L.or_calc_timestep.argtypes = [v, _dp, C.c_double, C.c_double, C.POINTER(C.c_int), _dp]
+    L.or_fill_cfl_dfl.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
+    L.or_rk_coefs.argtypes = [_dp, _dp, _dp]
+    L.or_timestep_lserkw2.argtypes = [v, C.c_double, C.c_double, _dp, _dp]
+    L.or_tgv_ic.argtypes = [P, C.c_double, C.c_int, _dp, _dp]
+    L.or_tgv_analyze.argtypes = [v, _dp, _dp, _dp, _dp, C.c_int, _dp]
+    L.or_prolong_to_face_raw.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp, _dp]
+    L.or_surf_int_raw.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp, C.c_double, _dp]
+
+
+def F(shape, dtype=np.float64):
+    return np.zeros(shape, dtype=dtype, order="F")
+
+
+def params(N, nodeType, splitDG, riemann, kappa=1.4, R=71.42857, Pr=0.72, mu0=6.25e-4) -> Params:
+    return Params(N, nodeType, splitDG, riemann, kappa, R, Pr, mu0)
+
+
+def nodes_weights(N, nodeType):
+    x, w, wb = F(N + 1), F(N + 1), F(N + 1)
+    lib().or_nodes_weights(N, nodeType, x, w, wb)
+    return x, w, wb
+
+
+def derivative_matrix(N, nodeType):
+    D = F((N + 1, N + 1))
+    lib().or_derivative_matrix(N, nodeType, D)
+    return D
+
+
+def vandermonde(Nin, tin, Nout, tout, modal=False):
+    V = F((Nout + 1, Nin + 1))
+    lib().or_vandermonde(Nin, tin, Nout, tout, int(modal), V)
+    return V
+
+
+def dg_basis(N, nodeType):
+    n = N + 1
+    out = dict(xGP=F(n), wGP=F(n), wBary=F(n), L_Minus=F(n), L_Plus=F(n), L_HatMinus=F(n),
+               L_HatPlus=F(n), D=F((n, n)), D_T=F((n, n)), D_Hat=F((n, n)), D_Hat_T=F((n, n)),
+               DVolSurf=F((n, n)))
+    lib().or_dg_basis(N, nodeType, *out.values())
+    return out
+
+
+def mappings(N):
+    n = N + 1
+    S2V2 = F((2, n, n, 5, 6), np.int32)
+    S2V = F((3, n, n, n, 5, 6), np.int32)
+    V2S = F((3, n, n, n, 5, 6), np.int32)
+    lib().or_mappings(N, S2V2, S2V, V2S)
+    return S2V2, S2V, V2S
+
+
+def calc_metrics(N, nodeType, mesh):
+    n = N + 1
+    K, S = mesh.nElems, mesh.nSides
+    out = dict(Metrics_fTilde=F((3, n, n, n, K)), Metrics_gTilde=F((3, n, n, n, K)),
+               Metrics_hTilde=F((3, n, n, n, K)), sJ=F((n, n, n, K)), Elem_xGP=F((3, n, n, n, K)),
+               NormVec=F((3, n, n, S)), TangVec1=F((3, n, n, S)), TangVec2=F((3, n, n, S)),
+               SurfElem=F((n, n, S)), Face_xGP=F((3, n, n, S)))
+    lib().or_calc_metrics(N, nodeType, mesh.NGeo, K, np.asfortranarray(mesh.NodeCoords), S,
+                          np.asfortranarray(mesh.ElemToSide), *out.values())
+    return out
+
+
+def fill_cfl_dfl(N, nodeType, CFL, DFL):
+    a, b = C.c_double(), C.c_double()
+    lib().or_fill_cfl_dfl(N, nodeType, CFL, DFL, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def rk_coefs():
+    A, B, Cc = F(5), F(5), F(5)
+    lib().or_rk_coefs(A, B, Cc)
+    return A, B, Cc
+
+
+class Solver:
+    """The oracle's DG operator on one mesh (FLEXI layout)."""
+
+    def __init__(self, prm: Params, mesh, metrics):
+        self.p = prm
+        self.N = prm.N
+        self.mesh = mesh
+        self.m = metrics
+        f = np.asfortranarray
+        self._keep = [f(metrics[k]) for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ",
+                                               "NormVec", "TangVec1", "TangVec2", "SurfElem")]
+        self.ctx = lib().or_init(C.byref(prm), mesh.nElems, mesh.nSides, f(mesh.SideToElem),
+                                 f(mesh.ElemToSide), *self._keep)
+        n = self.N + 1
+        self.shape = (5, n, n, n, mesh.nElems)
+
+    def __del__(self):
+        try:
+            lib().or_free(self.ctx)
+        except Exception:
+            pass
+
+    def rhs(self, U, t=0.0):
+        Ut = F(self.shape)
+        lib().or_dg_time_derivative(self.ctx, t, np.asfortranarray(U), Ut)
+        return Ut
+
+    def gradients(self):
+        n = self.N + 1
+        g = [F((4, n, n, n, self.mesh.nElems)) for _ in range(3)]
+        lib().or_get_gradients(self.ctx, *g)
+        return g
+
+    def faces(self):
+        n = self.N + 1
+        f = [F((5, n, n, self.mesh.nSides)) for _ in range(3)]
+        lib().or_get_faces(self.ctx, *f)
+        return f
+
+    def calc_timestep(self, U, CFL, DFL):
+        err = C.c_int()
+        dtElem = F(self.mesh.nElems)
+        dt = lib().or_calc_timestep(self.ctx, np.asfortranarray(U), CFL, DFL, C.byref(err), dtElem)
+        return dt, err.value, dtElem
+
+    def step(self, U, t, dt, R=None):
+        """One LSERK step in place on ``U`` (Fortran-ordered float64)."""
+        if R is None:
+            R = F(self.shape)
+        lib().or_timestep_lserkw2(self.ctx, t, dt, U, R)
+        return U
+
+    def tgv_analyze(self, U, grads, NAnalyze=10):
+        out = F(15)
+        lib().or_tgv_analyze(self.ctx, np.asfortranarray(U), *grads, NAnalyze, out)
+        return out
+
+
+def tgv_ic(prm: Params, xyz, Ma=0.1):
+    """TGV initial state at points ``xyz(3,...)`` (Fortran order) -> ``U(5,...)``."""
+    pts = np.asfortranarray(xyz.reshape(3, -1, order="F"))
+    U = F((5, pts.shape[1]))
+    lib().or_tgv_ic(C.byref(prm), Ma, pts.shape[1], pts, U)
+    return U.reshape((5,) + xyz.shape[1:], order="F")
+
+
+def prolong_to_face_raw(N, nodeType, S2E, L_Minus, L_Plus, Uvol):
+    nVar = Uvol.shape[0]
+    nSides = S2E.shape[1]
+    n = N + 1
+    Um, Us = F((nVar, n, n, nSides)), F((nVar, n, n, nSides))
+    lib().or_prolong_to_face_raw(N, nodeType, nSides, np.asfortranarray(S2E, dtype=np.int32),
+                                 np.asfortranarray(L_Minus), np.asfortranarray(L_Plus), nVar,
+                                 np.asfortranarray(Uvol), Um, Us)
+    return Um, Us
+
+
+def surf_int_raw(N, nodeType, S2E, L_HatMinus, L_HatPlus, Fm, Fs, nElems, slave_sign=-1.0):
+    nVar = Fm.shape[0]
+    n = N + 1
+    Ut = F((nVar, n, n, n, nElems))
+    lib().or_surf_int_raw(N, nodeType, S2E.shape[1], np.asfortranarray(S2E, dtype=np.int32),
+                          np.asfortranarray(L_HatMinus), np.asfortranarray(L_HatPlus), nVar,
+                          np.asfortranarray(Fm), np.asfortranarray(Fs), slave_sign, Ut)
+    return Ut
+
+
+def set_threads(n: int) -> None:
+    lib().or_set_threads(n)
