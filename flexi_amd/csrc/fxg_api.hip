@@ -1,0 +1,447 @@
+// Host side of libfxg: context set-up, the DG basis, device memory and the C ABI of include/fxg.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fxg.h"
+#include "fxg_basis.h"
+#include "fxg_kernels.h"
+#include "fxg_metrics.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return 1;
+}
+
+#define HIPCHECK(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Operators of InitDGBasis (src/dg/dg.f90:132-209) in the BasisLayout order.
+std::vector<double> build_basis(int N, int nodeType) {
+  const int n = N + 1;
+  std::vector<double> x(n), w(n), wb(n), D(n * n), Lm(n), Lp(n);
+  fxg::basis::nodes(N, nodeType == FXG_NODE_GAUSS ? fxg::basis::GAUSS : fxg::basis::GAUSS_LOBATTO, x.data(), w.data());
+  fxg::basis::bary_weights(N, x.data(), wb.data());
+  fxg::basis::diff_matrix(N, x.data(), D.data());
+  fxg::basis::lagrange_at(-1.0, N, x.data(), wb.data(), Lm.data());
+  fxg::basis::lagrange_at(1.0, N, x.data(), wb.data(), Lp.data());
+  fxg::BasisLayout B{n};
+  std::vector<double> out(B.size(), 0.0);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      out[B.D_T() + a + n * b] = D[b + n * a];
+      // D_Hat = -M^{-1} D^T M  ->  D_Hat(b,a) = -D(a,b) w(a)/w(b);  D_Hat_T(a,b) = D_Hat(b,a)
+      out[B.D_Hat_T() + a + n * b] = -(1.0 / w[b]) * (D[a + n * b] * w[a]);
+      out[B.DVolSurf() + a + n * b] = D[b + n * a];
+    }
+  if (nodeType == FXG_NODE_GAUSS_LOBATTO) {
+    // corner terms of the split-form operator cancel for Gauss-Lobatto (dg.f90:196-201)
+    out[B.DVolSurf() + 0] += 1.0 / (2.0 * w[0]);
+    out[B.DVolSurf() + N + n * N] -= 1.0 / (2.0 * w[N]);
+  }
+  for (int a = 0; a < n; ++a) {
+    out[B.L_Minus() + a] = Lm[a];
+    out[B.L_Plus() + a] = Lp[a];
+    out[B.L_HatMinus() + a] = (1.0 / w[a]) * Lm[a];
+    out[B.L_HatPlus() + a] = (1.0 / w[a]) * Lp[a];
+  }
+  return out;
+}
+
+// Side-to-volume index map S2V2 (src/mesh/mappings.f90:188-446) and its inverse.
+void flip_s2m(int N, int p, int q, int flip, int& a, int& b) {
+  switch (flip) {
+    case 0: a = p; b = q; break;
+    case 1: a = q; b = p; break;
+    case 2: a = N - p; b = q; break;
+    case 3: a = N - q; b = N - p; break;
+    default: a = p; b = N - q; break;
+  }
+}
+void cgns_s2v2(int N, int p, int q, int ls, int& a, int& b) {
+  switch (ls) {
+    case 5: a = q; b = p; break;
+    case 3: a = p; b = q; break;
+    case 2: a = p; b = q; break;
+    case 4: a = N - p; b = q; break;
+    case 1: a = q; b = p; break;
+    default: a = p; b = q; break;
+  }
+}
+void build_maps(int N, std::vector<int>& s2v2, std::vector<int>& v2s2) {
+  const int n = N + 1, nf = n * n;
+  s2v2.assign(2 * nf * 30, 0);
+  v2s2.assign(2 * nf * 30, 0);
+  for (int ls = 1; ls <= 6; ++ls)
+    for (int f = 0; f <= 4; ++f)
+      for (int q = 0; q <= N; ++q)
+        for (int p = 0; p <= N; ++p) {
+          int pp, qq, a, b;
+          flip_s2m(N, p, q, f, pp, qq);
+          cgns_s2v2(N, pp, qq, ls, a, b);
+          const int o = nf * (f + 5 * (ls - 1));
+          s2v2[2 * (p + n * q + o)] = a;
+          s2v2[2 * (p + n * q + o) + 1] = b;
+          v2s2[2 * (a + n * b + o)] = p;
+          v2s2[2 * (a + n * b + o) + 1] = q;
+        }
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+}  // namespace
+
+struct fxg_ctx {
+  fxg_params p{};
+  int device = 0;
+  int N = 0, n = 0, nf = 0, np = 0, K = 0, S = 0;
+  hipStream_t stream = nullptr;
+  fxg::Gas gas{};
+  fxg::DevMesh m{};
+  fxg::DevWork w{};
+  double *U = nullptr, *Ut = nullptr, *R = nullptr;
+  // owned device arrays
+  int *E2S = nullptr, *s2v2 = nullptr, *v2s2 = nullptr;
+  double *Mf = nullptr, *Mg = nullptr, *Mh = nullptr, *sJ = nullptr, *NV = nullptr, *T1 = nullptr, *T2 = nullptr,
+         *SE = nullptr, *adv = nullptr, *visc = nullptr, *basis = nullptr, *xGP = nullptr;
+  double *dtHost = nullptr;
+  int* errHost = nullptr;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+int check_params(const fxg_params* p) {
+  if (!p) return fail("params is NULL");
+  if (!fxg::supported_degree(p->N)) return fail("N must be between 1 and 7");
+  if (p->nodeType != FXG_NODE_GAUSS && p->nodeType != FXG_NODE_GAUSS_LOBATTO) return fail("unknown nodeType");
+  if (p->splitDG != FXG_SPLITDG_NONE && p->splitDG != FXG_SPLITDG_PI) return fail("SplitDG must be 0 (weak) or PI (4)");
+  if (p->splitDG == FXG_SPLITDG_PI && p->nodeType != FXG_NODE_GAUSS_LOBATTO)
+    return fail("the split form is built for Gauss-Lobatto nodes only");
+  if (p->riemann != FXG_RIEMANN_LF && p->riemann != FXG_RIEMANN_ROE && p->riemann != FXG_RIEMANN_ROEENTROPYFIX)
+    return fail("Riemann must be LF (1), Roe (3) or RoeEntropyFix (33)");
+  if (p->doWeakLifting || p->doConservativeLifting)
+    return fail("only the strong, non-conservative BR1 lifting is built (doWeakLifting=F, doConservativeLifting=F)");
+  if (p->precision != FXG_PREC_FP64 && p->precision != FXG_PREC_FP32_BULK) return fail("unknown precision");
+  if (!(p->kappa > 1.0) || !(p->R > 0.0) || !(p->Pr > 0.0) || !(p->mu0 >= 0.0)) return fail("invalid gas constants");
+  return 0;
+}
+
+// Connectivity must be a closed conforming mesh: every side has one master (flip 0) and one
+// slave, consistent between SideToElem and ElemToSide. Checked on the host so that no kernel
+// can index out of bounds.
+int check_connectivity(int K, int S, const int* S2E, const int* E2S) {
+  if (K <= 0 || S <= 0) return fail("nElems and nSides must be positive");
+  if (!S2E || !E2S) return fail("SideToElem/ElemToSide is NULL");
+  std::vector<int> seen(2 * (size_t)S, 0);
+  for (int e = 0; e < K; ++e)
+    for (int ls = 1; ls <= 6; ++ls) {
+      const int sid = E2S[2 * (ls - 1) + 12 * e], flip = E2S[2 * (ls - 1) + 1 + 12 * e];
+      if (sid < 1 || sid > S) return fail("ElemToSide side id out of range");
+      if (flip < 0 || flip > 4) return fail("ElemToSide flip out of range");
+      const int* se = S2E + 5 * (sid - 1);
+      if (flip == 0) {
+        if (se[0] != e + 1 || se[2] != ls) return fail("ElemToSide/SideToElem master mismatch");
+        seen[2 * (sid - 1)]++;
+      } else {
+        if (se[1] != e + 1 || se[3] != ls || se[4] != flip) return fail("ElemToSide/SideToElem slave mismatch");
+        seen[2 * (sid - 1) + 1]++;
+      }
+    }
+  for (int s = 0; s < S; ++s)
+    if (seen[2 * s] != 1 || seen[2 * s + 1] != 1)
+      return fail("every side needs exactly one master and one slave (periodic, conforming mesh)");
+  return 0;
+}
+
+template <class T>
+int upload(fxg_ctx* c, T** dst, const T* src, size_t count) {
+  if (dev_alloc(dst, count)) return 1;
+  c->owned.push_back(*dst);
+  if (src) {
+    hipError_t e = hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+template <class T>
+int alloc_owned(fxg_ctx* c, T** dst, size_t count) { return upload<T>(c, dst, nullptr, count); }
+
+// Allocations and tables shared by both creation paths. Geometry arrays are allocated but filled
+// by the caller.
+int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, const int* E2S) {
+  c->p = *p;
+  HIPCHECK(hipGetDevice(&c->device));
+  c->N = p->N; c->n = p->N + 1; c->nf = c->n * c->n; c->np = c->nf * c->n; c->K = K; c->S = S;
+  HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const double kappaM1 = p->kappa - 1.0;
+  const double cp = p->R * p->kappa / kappaM1;
+  c->gas = fxg::Gas{p->kappa, kappaM1, 1.0 / kappaM1, p->R, 1.0 / p->R, p->mu0, p->mu0 * cp / p->Pr, 4.0 / 3.0, 2.0 / 3.0};
+  const size_t vol = (size_t)c->np * K, face = (size_t)c->nf * S;
+  std::vector<int> s2v2, v2s2;
+  build_maps(c->N, s2v2, v2s2);
+  std::vector<double> basis = build_basis(c->N, p->nodeType);
+  if (upload(c, &c->E2S, E2S, (size_t)12 * K) || upload(c, &c->s2v2, s2v2.data(), s2v2.size()) ||
+      upload(c, &c->v2s2, v2s2.data(), v2s2.size()) || upload(c, &c->basis, basis.data(), basis.size()))
+    return 1;
+  if (alloc_owned(c, &c->Mf, 3 * vol) || alloc_owned(c, &c->Mg, 3 * vol) || alloc_owned(c, &c->Mh, 3 * vol) ||
+      alloc_owned(c, &c->sJ, vol) || alloc_owned(c, &c->NV, 3 * face) || alloc_owned(c, &c->T1, 3 * face) ||
+      alloc_owned(c, &c->T2, 3 * face) || alloc_owned(c, &c->SE, face) || alloc_owned(c, &c->adv, 3 * vol) ||
+      alloc_owned(c, &c->visc, 3 * vol) || alloc_owned(c, &c->xGP, 3 * vol))
+    return 1;
+  if (alloc_owned(c, &c->U, 5 * vol) || alloc_owned(c, &c->Ut, 5 * vol) || alloc_owned(c, &c->R, 5 * vol)) return 1;
+  HIPCHECK(hipMemsetAsync(c->U, 0, 5 * vol * sizeof(double), c->stream));
+  HIPCHECK(hipMemsetAsync(c->Ut, 0, 5 * vol * sizeof(double), c->stream));
+  HIPCHECK(hipMemsetAsync(c->R, 0, 5 * vol * sizeof(double), c->stream));
+  const size_t gsz = p->precision == FXG_PREC_FP32_BULK ? sizeof(float) : sizeof(double);
+  char *G, *Gm, *Gs;
+  if (alloc_owned(c, &c->w.Uf_m, 5 * face) || alloc_owned(c, &c->w.Uf_s, 5 * face) ||
+      alloc_owned(c, &G, 12 * vol * gsz) || alloc_owned(c, &Gm, 12 * face * gsz) || alloc_owned(c, &Gs, 12 * face * gsz) ||
+      alloc_owned(c, &c->w.Flux, 5 * face) || alloc_owned(c, &c->w.dtElem, 2 * (size_t)K) ||
+      alloc_owned(c, &c->w.errElem, (size_t)K) || alloc_owned(c, &c->w.dtOut, 2) || alloc_owned(c, &c->w.errOut, 1))
+    return 1;
+  c->w.G = G; c->w.Gf_m = Gm; c->w.Gf_s = Gs;
+  HIPCHECK(hipMemsetAsync(G, 0, 12 * vol * gsz, c->stream));
+  HIPCHECK(hipHostMalloc((void**)&c->dtHost, 2 * sizeof(double), hipHostMallocDefault));
+  HIPCHECK(hipHostMalloc((void**)&c->errHost, sizeof(int), hipHostMallocDefault));
+  c->m = fxg::DevMesh{c->N, K, S, c->E2S, c->s2v2, c->v2s2, c->Mf, c->Mg, c->Mh, c->sJ,
+                      c->NV, c->T1, c->T2, c->SE, c->adv, c->visc, c->basis};
+  return 0;
+}
+
+int finish_setup(fxg_ctx* c) {
+  const double KappasPr = c->p.kappa / c->p.Pr;
+  const double kmax = std::fmax(4.0 / 3.0, KappasPr);  // KAPPASPR_MAX_TIMESTEP_H, eos.h
+  HIPCHECK(fxg::launch_metrics_adv(c->m, kmax, c->adv, c->visc, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+void ctx_free(fxg_ctx* c) {
+  if (!c) return;
+  for (void* ptr : c->owned) (void)hipFree(ptr);
+  if (c->dtHost) (void)hipHostFree(c->dtHost);
+  if (c->errHost) (void)hipHostFree(c->errHost);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStream_t s) {
+  HIPCHECK(fxg::launch_rhs(c->m, c->w, c->gas, c->p.nodeType, c->p.splitDG == FXG_SPLITDG_PI, c->p.riemann,
+                           c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st, s));
+  return 0;
+}
+
+int lserk(fxg_ctx* c, double dt, int nStages, const double* RKA, const double* RKb, double* U, double* R,
+          hipStream_t s) {
+  if (nStages < 1) return fail("nStages must be positive");
+  if (!RKb || (nStages > 1 && !RKA)) return fail("RK coefficient arrays are NULL");
+  for (int st = 0; st < nStages; ++st) {
+    fxg::RKStage stage{st == 0 ? 1 : 2, st == 0 ? 0.0 : -RKA[st - 1], RKb[st] * dt, U, R};
+    if (rhs(c, U, nullptr, stage, s)) return 1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fxg_last_error(void) { return g_err.c_str(); }
+const char* fxg_version(void) { return "fxg 0.1 (gfx950)"; }
+
+int fxg_create(const fxg_params* p, const fxg_mesh* mesh, fxg_ctx** out) {
+  if (!out) return fail("ctx output pointer is NULL");
+  *out = nullptr;
+  if (check_params(p)) return 1;
+  if (!mesh) return fail("mesh is NULL");
+  if (mesh->firstInnerSide != 1 || mesh->lastInnerSide != mesh->nSides)
+    return fail("only meshes without boundary or MPI sides are supported (firstInnerSide=1, lastInnerSide=nSides)");
+  if (check_connectivity(mesh->nElems, mesh->nSides, mesh->SideToElem, mesh->ElemToSide)) return 1;
+  if (!mesh->Metrics_fTilde || !mesh->Metrics_gTilde || !mesh->Metrics_hTilde || !mesh->sJ || !mesh->NormVec ||
+      !mesh->TangVec1 || !mesh->TangVec2 || !mesh->SurfElem)
+    return fail("a metric array is NULL");
+  fxg_ctx* c = new fxg_ctx;
+  if (ctx_setup(c, p, mesh->nElems, mesh->nSides, mesh->SideToElem, mesh->ElemToSide)) { ctx_free(c); return 1; }
+  const size_t vol = (size_t)c->np * c->K, face = (size_t)c->nf * c->S;
+  struct { double* d; const double* h; size_t n; } cp[] = {
+      {c->Mf, mesh->Metrics_fTilde, 3 * vol}, {c->Mg, mesh->Metrics_gTilde, 3 * vol}, {c->Mh, mesh->Metrics_hTilde, 3 * vol},
+      {c->sJ, mesh->sJ, vol}, {c->NV, mesh->NormVec, 3 * face}, {c->T1, mesh->TangVec1, 3 * face},
+      {c->T2, mesh->TangVec2, 3 * face}, {c->SE, mesh->SurfElem, face}};
+  for (auto& x : cp) {
+    hipError_t e = hipMemcpy(x.d, x.h, x.n * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { fail(std::string("hipMemcpy: ") + hipGetErrorString(e)); ctx_free(c); return 1; }
+  }
+  if (finish_setup(c)) { ctx_free(c); return 1; }
+  *out = c;
+  return 0;
+}
+
+int fxg_create_from_nodes(const fxg_params* p, int nElems, int nSides, const int* S2E, const int* E2S, int NGeo,
+                          const double* NodeCoords, fxg_ctx** out) {
+  if (!out) return fail("ctx output pointer is NULL");
+  *out = nullptr;
+  if (check_params(p)) return 1;
+  if (check_connectivity(nElems, nSides, S2E, E2S)) return 1;
+  if (NGeo < 1 || NGeo > 8) return fail("NGeo must be between 1 and 8");
+  if (!NodeCoords) return fail("NodeCoords is NULL");
+  fxg_ctx* c = new fxg_ctx;
+  if (ctx_setup(c, p, nElems, nSides, S2E, E2S)) { ctx_free(c); return 1; }
+  const size_t ng = (size_t)(NGeo + 1) * (NGeo + 1) * (NGeo + 1);
+  double* dNC = nullptr;
+  if (upload(c, &dNC, NodeCoords, 3 * ng * nElems)) { ctx_free(c); return 1; }
+  std::string err;
+  if (fxg::compute_metrics_gpu(c->N, p->nodeType, NGeo, nElems, dNC, c->m, c->Mf, c->Mg, c->Mh, c->sJ, c->xGP, c->NV,
+                               c->T1, c->T2, c->SE, c->stream, err)) {
+    fail(err); ctx_free(c); return 1;
+  }
+  if (finish_setup(c)) { ctx_free(c); return 1; }
+  *out = c;
+  return 0;
+}
+
+int fxg_destroy(fxg_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  ctx_free(c);
+  return 0;
+}
+
+int fxg_sizes(fxg_ctx* c, int* N, int* K, int* S) {
+  if (!c) return fail("ctx is NULL");
+  if (N) *N = c->N;
+  if (K) *K = c->K;
+  if (S) *S = c->S;
+  return 0;
+}
+
+int fxg_set_state(fxg_ctx* c, const double* U) {
+  if (!c || !U) return fail("NULL argument");
+  HIPCHECK(hipMemcpyAsync(c->U, U, 5 * (size_t)c->np * c->K * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fxg_get_state(fxg_ctx* c, double* U) {
+  if (!c || !U) return fail("NULL argument");
+  HIPCHECK(hipMemcpyAsync(U, c->U, 5 * (size_t)c->np * c->K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fxg_get_ut(fxg_ctx* c, double* Ut) {
+  if (!c || !Ut) return fail("NULL argument");
+  HIPCHECK(hipMemcpyAsync(Ut, c->Ut, 5 * (size_t)c->np * c->K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fxg_get_gradients(fxg_ctx* c, double* gx, double* gy, double* gz) {
+  if (!c || !gx || !gy || !gz) return fail("NULL argument");
+  const size_t nn = 4 * (size_t)c->np * c->K;
+  double* tmp = nullptr;
+  if (dev_alloc(&tmp, 3 * nn)) return 1;
+  hipError_t e = fxg::launch_unpack_gradients(c->m, c->w.G, c->p.precision == FXG_PREC_FP32_BULK, tmp, tmp + nn,
+                                              tmp + 2 * nn, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(gx, tmp, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(gy, tmp + nn, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(gz, tmp + 2 * nn, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(std::string("fxg_get_gradients: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int fxg_get_metrics(fxg_ctx* c, double* Mf, double* Mg, double* Mh, double* sJ, double* NV, double* T1, double* T2,
+                    double* SE, double* xGP) {
+  if (!c) return fail("ctx is NULL");
+  const size_t vol = (size_t)c->np * c->K, face = (size_t)c->nf * c->S;
+  struct { double* h; const double* d; size_t n; } cp[] = {
+      {Mf, c->Mf, 3 * vol}, {Mg, c->Mg, 3 * vol}, {Mh, c->Mh, 3 * vol}, {sJ, c->sJ, vol}, {NV, c->NV, 3 * face},
+      {T1, c->T1, 3 * face}, {T2, c->T2, 3 * face}, {SE, c->SE, face}, {xGP, c->xGP, 3 * vol}};
+  for (auto& x : cp)
+    if (x.h) HIPCHECK(hipMemcpyAsync(x.h, x.d, x.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fxg_dg_time_derivative(fxg_ctx* c, double t) {
+  (void)t;  // no time-dependent sources or boundaries on periodic meshes
+  if (!c) return fail("ctx is NULL");
+  fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
+  if (rhs(c, c->U, c->Ut, st, c->stream)) return 1;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fxg_timestep_lserkw2(fxg_ctx* c, double t, double dt, int nStages, const double* RKA, const double* RKb,
+                         const double* RKc) {
+  (void)t;
+  (void)RKc;  // stage times only enter time-dependent sources, which periodic flows do not have
+  if (!c) return fail("ctx is NULL");
+  if (lserk(c, dt, nStages, RKA, RKb, c->U, c->R, c->stream)) return 1;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fxg_dg_time_derivative_dev(fxg_ctx* c, double t, const double* U, double* Ut, void* stream) {
+  (void)t;
+  if (!c || !U || !Ut) return fail("NULL argument");
+  fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
+  return rhs(c, U, Ut, st, stream ? (hipStream_t)stream : c->stream);
+}
+
+int fxg_timestep_lserkw2_dev(fxg_ctx* c, double t, double dt, int nStages, const double* RKA, const double* RKb,
+                             const double* RKc, double* U, double* R, void* stream) {
+  (void)t;
+  (void)RKc;
+  if (!c || !U || !R) return fail("NULL argument");
+  return lserk(c, dt, nStages, RKA, RKb, U, R, stream ? (hipStream_t)stream : c->stream);
+}
+
+int fxg_sync(fxg_ctx* c) {
+  if (!c) return fail("ctx is NULL");
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fxg_calc_timestep(fxg_ctx* c, double CFL, double DFL, double* dt, int* errType, double* dtElem) {
+  if (!c || !dt || !errType) return fail("NULL argument");
+  HIPCHECK(fxg::launch_calc_timestep(c->m, c->w, c->gas, c->U, CFL, DFL, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->dtHost, c->w.dtOut, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->errHost, c->w.errOut, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (dtElem) {
+    // dtElem = MIN(convective, viscous) per element, as CALCTIMESTEP stores it
+    std::vector<double> tmp(2 * (size_t)c->K);
+    HIPCHECK(hipMemcpyAsync(tmp.data(), c->w.dtElem, 2 * (size_t)c->K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int e = 0; e < c->K; ++e) dtElem[e] = std::fmin(tmp[e], tmp[c->K + e]);
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  *dt = std::fmin(c->dtHost[0], c->dtHost[1]);
+  *errType = c->errHost[0];
+  if (*errType != 0) return fail("CALCTIMESTEP: NaN in density (1), convective (2) or viscous (3) time step: " +
+                                 std::to_string(*errType));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Launch interface between the host context (fxg_api.hip) and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fxg_physics.h"
+
+namespace fxg {
+
+// Offsets (in doubles) of the 1D operators inside the device basis buffer; every block is
+// (N+1)^2 or (N+1) long. Column-major like FLEXI: M(a,b) at a + (N+1)*b.
+struct BasisLayout {
+  int n;
+  __host__ __device__ int D_T() const { return 0; }
+  __host__ __device__ int D_Hat_T() const { return n * n; }
+  __host__ __device__ int DVolSurf() const { return 2 * n * n; }
+  __host__ __device__ int L_Minus() const { return 3 * n * n; }
+  __host__ __device__ int L_Plus() const { return 3 * n * n + n; }
+  __host__ __device__ int L_HatMinus() const { return 3 * n * n + 2 * n; }
+  __host__ __device__ int L_HatPlus() const { return 3 * n * n + 3 * n; }
+  __host__ __device__ int size() const { return 3 * n * n + 4 * n; }
+};
+
+// Device views of one context's arrays (FLEXI layouts unless noted).
+struct DevMesh {
+  int N, nElems, nSides;
+  const int* E2S;    // ElemToSide(2,6,nElems), 1-based side ids
+  const int* s2v2;   // S2V2(2,0:N,0:N,0:4,1:6)
+  const int* v2s2;   // inverse of S2V2: volume-face (a,b) -> side point (p,q), same shape
+  const double* Mf;  // Metrics_fTilde(3,0:N,0:N,0:N,nElems)
+  const double* Mg;
+  const double* Mh;
+  const double* sJ;
+  const double* NormVec;  // (3,0:N,0:N,nSides)
+  const double* TangVec1;
+  const double* TangVec2;
+  const double* SurfElem;  // (0:N,0:N,nSides)
+  const double* MetricsAdv;   // (3,0:N,0:N,0:N,nElems), calctimestep.f90:42-83
+  const double* MetricsVisc;  // (3,0:N,0:N,0:N,nElems)
+  const double* basis;        // BasisLayout
+};
+
+// Work arrays of one right-hand side evaluation. Face arrays of U are FP64 (U(5,p,q,side));
+// gradient arrays hold T = double or float (FP32-bulk).
+struct DevWork {
+  double* Uf_m;    // (5,0:N,0:N,nSides)
+  double* Uf_s;
+  void* G;         // volume gradients: (0:N^3, 12, nElems), components gx(u,v,w,T),gy(..),gz(..)
+  void* Gf_m;      // face gradients (12,0:N,0:N,nSides)
+  void* Gf_s;
+  double* Flux;    // Flux_master(5,0:N,0:N,nSides); Flux_slave equals it (fillflux.f90:199)
+  double* dtElem;  // (nElems)
+  int* errElem;    // (nElems)
+  double* dtOut;   // [2]: convective and viscous minima
+  int* errOut;     // [1]
+};
+
+struct RKStage {
+  int mode;        // 0: write Ut; 1: LSERK register update (first stage); 2: later stages
+  double minusA;   // -RKA(iStage)
+  double b_dt;     // RKb(iStage)*dt
+  double* U;       // state, updated in place
+  double* R;       // LSERK register Ut_tmp
+};
+
+// Runs prolongation, lifting, fluxes and the volume/surface integrals for one RHS.
+// Ut may be null when stage.mode != 0.
+hipError_t launch_rhs(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, int split,
+                      int riemann, int fp32, const double* U, double* Ut, const RKStage& stage,
+                      hipStream_t s);
+hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U,
+                                double CFL, double DFL, hipStream_t s);
+hipError_t launch_metrics_adv(const DevMesh& m, double KappasPr_max, double* adv, double* visc,
+                              hipStream_t s);
+// Copies a (12, np, nElems) gradient array of type float/double into three FLEXI-layout
+// FP64 arrays (4,0:N,0:N,0:N,nElems).
+hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, double* gx, double* gy,
+                                   double* gz, hipStream_t s);
+bool supported_degree(int N);
+
+}  // namespace fxg

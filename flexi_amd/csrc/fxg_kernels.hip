@@ -1,0 +1,677 @@
+// gfx950 kernels of the DGSEM right-hand side (FLEXI's DGTimeDerivative_weakForm) with BR1
+// lifting, the fused LSERK register update and the CFL time step.
+//
+// Element kernels run one workgroup per element with one lane per volume node (N+1)^3, rounded
+// up to whole 64-lane wavefronts; element data are staged through LDS with coalesced loads.
+// Side kernels run one lane per side point. No atomics: every element gathers the fluxes of its
+// own six sides, so results do not depend on scheduling.
+#include "fxg_kernels.h"
+
+namespace fxg {
+
+template <int N>
+struct Dim {
+  static constexpr int n = N + 1;
+  static constexpr int nf = n * n;
+  static constexpr int np = n * n * n;
+  static constexpr int NT = ((np + 63) / 64) * 64;
+};
+
+// Local side ids of flexi.h: 1 zeta-, 2 eta-, 3 xi+, 4 eta+, 5 xi-, 6 zeta+.
+__device__ __forceinline__ bool is_minus(int ls) { return ls == 5 || ls == 2 || ls == 1; }
+__device__ __forceinline__ int side_dir(int ls) { return (ls == 5 || ls == 3) ? 0 : (ls == 2 || ls == 4) ? 1 : 2; }
+
+// Volume node of local side ls at face coordinates (a,b) and normal coordinate l; the face
+// coordinates follow EvalElemFaceG/DoSurfInt (prolongtoface.t90:211-264, surfint.t90:683-766).
+template <int N>
+__device__ __forceinline__ int vol_index(int ls, int a, int b, int l) {
+  constexpr int n = N + 1;
+  switch (side_dir(ls)) {
+    case 0: return l + n * (a + n * b);
+    case 1: return a + n * (l + n * b);
+    default: return a + n * (b + n * l);
+  }
+}
+
+// Face coordinates (a,b) and normal coordinate l of node (i,j,k) with respect to local side ls.
+__device__ __forceinline__ void face_coords(int ls, int i, int j, int k, int& a, int& b, int& l) {
+  switch (side_dir(ls)) {
+    case 0: a = j; b = k; l = i; break;
+    case 1: a = i; b = k; l = j; break;
+    default: a = i; b = j; l = k; break;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void load_basis(const double* __restrict__ g, double* s, int t, int nt) {
+  constexpr int n = N + 1;
+  for (int x = t; x < 3 * n * n + 4 * n; x += nt) s[x] = g[x];
+}
+
+// ---------------------------------------------------------------------------------------------
+// ProlongToFace of the conservative state (prolongtoface.t90:21-144): every element writes the
+// face values of its six sides into the master or slave face array of the side, reordered into
+// the side frame with S2V2.
+template <int N, bool GL>
+__global__ __launch_bounds__(Dim<N>::NT) void k_prolong_u(const double* __restrict__ U, DevMesh m,
+                                                        double* __restrict__ Um, double* __restrict__ Us) {
+  constexpr int n = Dim<N>::n, nf = Dim<N>::nf, np = Dim<N>::np, NT = Dim<N>::NT;
+  __shared__ double sU[5 * np];
+  __shared__ double sB[3 * n * n + 4 * n];
+  const int e = blockIdx.x;
+  const int t = threadIdx.x;
+  const double* Ue = U + (size_t)5 * np * e;
+  for (int x = t; x < 5 * np; x += NT) sU[x] = Ue[x];
+  load_basis<N>(m.basis, sB, t, NT);
+  __syncthreads();
+  const BasisLayout B{n};
+  for (int x = t; x < 6 * nf; x += NT) {
+    const int ls = x / nf + 1, pq = x - (ls - 1) * nf;
+    const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
+    const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
+    const int* ab = m.s2v2 + 2 * (pq + nf * (flip + 5 * (ls - 1)));
+    double out[5];
+    if (GL) {
+      const int id = vol_index<N>(ls, ab[0], ab[1], is_minus(ls) ? 0 : N);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] = sU[5 * id + v];
+    } else {
+      const double* L = sB + (is_minus(ls) ? B.L_Minus() : B.L_Plus());
+      const int id0 = vol_index<N>(ls, ab[0], ab[1], 0);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] = sU[5 * id0 + v] * L[0];
+      for (int l = 1; l <= N; ++l) {
+        const int id = vol_index<N>(ls, ab[0], ab[1], l);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) out[v] = out[v] + sU[5 * id + v] * L[l];
+      }
+    }
+    double* dst = (flip == 0 ? Um : Us) + 5 * ((size_t)pq + (size_t)nf * side);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) dst[v] = out[v];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// BR1 lifting (Lifting_BR1, lifting_br1.t90:48-181; strong form, non-conservative volume
+// integral lifting_volint.t90:26-118; surface flux lifting_fillflux.t90:40-239), Jacobian,
+// and prolongation of the lifted gradients to the element's six sides.
+template <int N, bool GL, class RF>
+__global__ __launch_bounds__(Dim<N>::NT) void k_lift(const double* __restrict__ U, DevMesh m,
+                                                   const double* __restrict__ Um, const double* __restrict__ Us,
+                                                   Gas gas, RF* __restrict__ G, RF* __restrict__ Gm,
+                                                   RF* __restrict__ Gs) {
+  constexpr int n = Dim<N>::n, nf = Dim<N>::nf, np = Dim<N>::np, NT = Dim<N>::NT;
+  __shared__ double sA[12 * np];
+  __shared__ double sB[3 * n * n + 4 * n];
+  const BasisLayout B{n};
+  const int e = blockIdx.x;
+  const int t = threadIdx.x;
+  const double* Ue = U + (size_t)5 * np * e;
+  for (int x = t; x < 5 * np; x += NT) sA[x] = Ue[x];
+  load_basis<N>(m.basis, sB, t, NT);
+  __syncthreads();
+  double* sP = sA + 5 * np;  // lifting variables (u,v,w,T), SoA
+  if (t < np) {
+    const Prim q = cons_to_prim(sA + 5 * t, gas);
+    sP[t] = q.u;
+    sP[np + t] = q.v;
+    sP[2 * np + t] = q.w;
+    sP[3 * np + t] = q.T;
+  }
+  __syncthreads();
+  RF g[12];
+  if (t < np) {
+    const int i = t % n, j = (t / n) % n, k = t / (n * n);
+    const double* DT = sB + B.D_T();
+    RF gxi[4], geta[4], gzeta[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      gxi[v] = (RF)DT[0 + n * i] * (RF)sP[v * np + (0 + n * (j + n * k))];
+      geta[v] = (RF)DT[0 + n * j] * (RF)sP[v * np + (i + n * (0 + n * k))];
+      gzeta[v] = (RF)DT[0 + n * k] * (RF)sP[v * np + (i + n * (j + n * 0))];
+    }
+    for (int l = 1; l <= N; ++l) {
+      const RF dxi = (RF)DT[l + n * i], deta = (RF)DT[l + n * j], dzeta = (RF)DT[l + n * k];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        gxi[v] = gxi[v] + dxi * (RF)sP[v * np + (l + n * (j + n * k))];
+        geta[v] = geta[v] + deta * (RF)sP[v * np + (i + n * (l + n * k))];
+        gzeta[v] = gzeta[v] + dzeta * (RF)sP[v * np + (i + n * (j + n * l))];
+      }
+    }
+    const size_t gid = (size_t)np * e + t;
+    RF mf[3], mg[3], mh[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      mf[c] = (RF)m.Mf[3 * gid + c];
+      mg[c] = (RF)m.Mg[3 * gid + c];
+      mh[c] = (RF)m.Mh[3 * gid + c];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) g[4 * d + v] = mf[d] * gxi[v] + mg[d] * geta[v] + mh[d] * gzeta[v];
+    // surface contributions: 0.5*(U_slave - U_master)*SurfElem*n, same sign on both sides
+#pragma unroll 1
+    for (int ls = 1; ls <= 6; ++ls) {
+      int a, b, l;
+      face_coords(ls, i, j, k, a, b, l);
+      if (GL && l != (is_minus(ls) ? 0 : N)) continue;
+      const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
+      const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
+      const int* pq2 = m.v2s2 + 2 * ((a + n * b) + nf * (flip + 5 * (ls - 1)));
+      const size_t fp = (size_t)(pq2[0] + n * pq2[1]) + (size_t)nf * side;
+      const Prim pm = cons_to_prim(Um + 5 * fp, gas);
+      const Prim ps = cons_to_prim(Us + 5 * fp, gas);
+      const RF se = (RF)m.SurfElem[fp];
+      const RF nv[3] = {(RF)m.NormVec[3 * fp], (RF)m.NormVec[3 * fp + 1], (RF)m.NormVec[3 * fp + 2]};
+      const RF lh = GL ? (RF)(is_minus(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N])
+                       : (RF)(is_minus(ls) ? sB[B.L_HatMinus() + l] : sB[B.L_HatPlus() + l]);
+      const RF dq[4] = {(RF)ps.u - (RF)pm.u, (RF)ps.v - (RF)pm.v, (RF)ps.w - (RF)pm.w, (RF)ps.T - (RF)pm.T};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const RF f = RF(0.5) * dq[v] * se;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[4 * d + v] = g[4 * d + v] + f * nv[d] * lh;
+      }
+    }
+    const RF sj = (RF)m.sJ[gid];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) g[c] = g[c] * sj;
+  }
+  __syncthreads();  // all reads of sP are done before sA is overwritten
+  if (t < np) {
+    RF* Ge = G + (size_t)12 * np * e;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      Ge[c * np + t] = g[c];
+      sA[c * np + t] = (double)g[c];
+    }
+  }
+  __syncthreads();
+  // ProlongToFaceLifting of the gradients into the side arrays (12 values per side point)
+  for (int x = t; x < 6 * nf; x += NT) {
+    const int ls = x / nf + 1, pq = x - (ls - 1) * nf;
+    const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
+    const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
+    const int* ab = m.s2v2 + 2 * (pq + nf * (flip + 5 * (ls - 1)));
+    RF* dst = (flip == 0 ? Gm : Gs) + 12 * ((size_t)pq + (size_t)nf * side);
+    if (GL) {
+      const int id = vol_index<N>(ls, ab[0], ab[1], is_minus(ls) ? 0 : N);
+#pragma unroll
+      for (int c = 0; c < 12; ++c) dst[c] = (RF)sA[c * np + id];
+    } else {
+      const double* L = sB + (is_minus(ls) ? B.L_Minus() : B.L_Plus());
+      const int id0 = vol_index<N>(ls, ab[0], ab[1], 0);
+      RF out[12];
+#pragma unroll
+      for (int c = 0; c < 12; ++c) out[c] = (RF)sA[c * np + id0] * (RF)L[0];
+      for (int l = 1; l <= N; ++l) {
+        const int id = vol_index<N>(ls, ab[0], ab[1], l);
+#pragma unroll
+        for (int c = 0; c < 12; ++c) out[c] = out[c] + (RF)sA[c * np + id] * (RF)L[l];
+      }
+#pragma unroll
+      for (int c = 0; c < 12; ++c) dst[c] = out[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FillFlux for inner sides (fillflux.f90:40-215): Riemann_Side + ViscousFlux_Side, times
+// SurfElem. One lane per side point.
+template <class RF, bool SPLIT>
+__global__ __launch_bounds__(256) void k_flux(DevMesh m, const double* __restrict__ Um, const double* __restrict__ Us,
+                                            const RF* __restrict__ Gm, const RF* __restrict__ Gs, Gas gas,
+                                            int solver, double* __restrict__ Flux, int nPts) {
+  const int fp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fp >= nPts) return;
+  const GasT<RF> g(gas);
+  const Prim pl = cons_to_prim(Um + 5 * (size_t)fp, gas);
+  const Prim pr = cons_to_prim(Us + 5 * (size_t)fp, gas);
+  RF nv[3], t1[3], t2[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    nv[c] = (RF)m.NormVec[3 * (size_t)fp + c];
+    t1[c] = (RF)m.TangVec1[3 * (size_t)fp + c];
+    t2[c] = (RF)m.TangVec2[3 * (size_t)fp + c];
+  }
+  const RotState<RF> L = rotate<RF>((RF)pl.rho, (RF)Um[5 * (size_t)fp + 4], (RF)pl.p, (RF)pl.u, (RF)pl.v, (RF)pl.w, nv, t1, t2);
+  const RotState<RF> R = rotate<RF>((RF)pr.rho, (RF)Us[5 * (size_t)fp + 4], (RF)pr.p, (RF)pr.u, (RF)pr.v, (RF)pr.w, nv, t1, t2);
+  RF F[5];
+  riemann_rot<RF>(g, solver, SPLIT, L, R, F);
+  RF Fo[5];
+  Fo[0] = F[0];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) Fo[1 + c] = nv[c] * F[1] + t1[c] * F[2] + t2[c] * F[3];
+  Fo[4] = F[4];
+  // ViscousFlux_Side (riemann.f90:448-498): 0.5*(f_L + f_R).n
+  RF gl[12], gr[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    gl[c] = Gm[12 * (size_t)fp + c];
+    gr[c] = Gs[12 * (size_t)fp + c];
+  }
+  RF fxl[4], fyl[4], fzl[4], fxr[4], fyr[4], fzr[4];
+  diff_flux<RF>(g, (RF)pl.u, (RF)pl.v, (RF)pl.w, gl, gl + 4, gl + 8, fxl, fyl, fzl);
+  diff_flux<RF>(g, (RF)pr.u, (RF)pr.v, (RF)pr.w, gr, gr + 4, gr + 8, fxr, fyr, fzr);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    Fo[1 + k] = Fo[1 + k] + RF(0.5) * (nv[0] * (fxl[k] + fxr[k]) + nv[1] * (fyl[k] + fyr[k]) + nv[2] * (fzl[k] + fzr[k]));
+  const double se = m.SurfElem[fp];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Flux[5 * (size_t)fp + v] = (double)Fo[v] * se;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Volume integral (VolInt_weakForm volint.f90:117-187, or VolInt_splitForm :211-371 with the
+// Pirozzoli flux, splitflux.f90:603-666), surface integral (surfint.t90), Ut=-Ut, Jacobian
+// (dg.f90:656-676), and optionally the LSERK update of TimeStepByLSERKW2 (timestep.f90:124-131).
+//
+// The split flux is evaluated in contravariant form: with m = (Ja_i + Ja_l)/2 and the sums
+// rho_s, v_s, p_s, H_s of the two states, F = (rho_s m.v_s/2, rho_s m.v_s v_s/4 + p_s m,
+// rho_s m.v_s H_s/4), algebraically identical to the reference's m.(f,g,h).
+template <int N, bool GL, bool SPLIT, class RF>
+__global__ __launch_bounds__(Dim<N>::NT) void k_volint(const double* __restrict__ U, DevMesh m,
+                                                     const RF* __restrict__ G, const double* __restrict__ Flux,
+                                                     Gas gas, double* __restrict__ Ut, RKStage st) {
+  constexpr int n = Dim<N>::n, nf = Dim<N>::nf, np = Dim<N>::np, NT = Dim<N>::NT;
+  constexpr int NPRIM = SPLIT ? 6 : 5;   // split: rho,u,v,w,p,H ; weak: staging of f~ (5 comps)
+  constexpr int NDIR = SPLIT ? 7 : 0;    // split: Ja^d (3) + viscous contravariant flux (4)
+  __shared__ double sA[(NPRIM + NDIR) * np > 5 * np ? (NPRIM + NDIR) * np : 5 * np];
+  __shared__ double sB[3 * n * n + 4 * n];
+  const BasisLayout B{n};
+  const GasT<RF> g(gas);
+  const int e = blockIdx.x;
+  const int t = threadIdx.x;
+  const double* Ue = U + (size_t)5 * np * e;
+  for (int x = t; x < 5 * np; x += NT) sA[x] = Ue[x];
+  load_basis<N>(m.basis, sB, t, NT);
+  __syncthreads();
+  const bool act = t < np;
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  double Uown[5] = {0, 0, 0, 0, 0};
+  Prim q{};
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Uown[v] = sA[5 * t + v];
+    q = cons_to_prim(Uown, gas);
+  }
+  // viscous Cartesian fluxes at the own node (EvalDiffFlux3D_Point)
+  RF fvx[4] = {0, 0, 0, 0}, fvy[4] = {0, 0, 0, 0}, fvz[4] = {0, 0, 0, 0};
+  RF Ja[9];
+  const size_t gid = (size_t)np * e + (act ? t : 0);
+  if (act) {
+    RF gg[12];
+    const RF* Ge = G + (size_t)12 * np * e;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) gg[c] = Ge[c * np + t];
+    diff_flux<RF>(g, (RF)q.u, (RF)q.v, (RF)q.w, gg, gg + 4, gg + 8, fvx, fvy, fvz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Ja[c] = (RF)m.Mf[3 * gid + c];
+      Ja[3 + c] = (RF)m.Mg[3 * gid + c];
+      Ja[6 + c] = (RF)m.Mh[3 * gid + c];
+    }
+  }
+  __syncthreads();  // sA (U staging) is reused below
+  RF acc[5] = {0, 0, 0, 0, 0};
+  if (SPLIT) {
+    double* sR = sA;           // rho
+    double* sUu = sA + np;     // u
+    double* sV = sA + 2 * np;  // v
+    double* sW = sA + 3 * np;  // w
+    double* sPp = sA + 4 * np; // p
+    double* sH = sA + 5 * np;  // H = (E+p)/rho
+    double* sM = sA + 6 * np;  // Ja^d (3 x np)
+    double* sF = sA + 9 * np;  // viscous contravariant flux (4 x np)
+    if (act) {
+      sR[t] = q.rho; sUu[t] = q.u; sV[t] = q.v; sW[t] = q.w; sPp[t] = q.p;
+      sH[t] = (Uown[4] + q.p) / Uown[0];
+    }
+    const RF rho = (RF)q.rho, uu = (RF)q.u, vv = (RF)q.v, ww = (RF)q.w, pp = (RF)q.p;
+    const RF HH = act ? (RF)((Uown[4] + q.p) / Uown[0]) : RF(0);
+    const double* DV = sB + B.DVolSurf();
+    const double* DH = sB + B.D_Hat_T();
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+      if (act) {
+        const RF* Jd = Ja + 3 * d;
+        sM[t] = (double)Jd[0]; sM[np + t] = (double)Jd[1]; sM[2 * np + t] = (double)Jd[2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sF[c * np + t] = (double)(Jd[0] * fvx[c] + Jd[1] * fvy[c] + Jd[2] * fvz[c]);
+      }
+      __syncthreads();
+      if (act) {
+        const int me = d == 0 ? i : (d == 1 ? j : k);
+        const int stride = d == 0 ? 1 : (d == 1 ? n : n * n);
+        const int base = t - me * stride;
+        const RF m0 = Ja[3 * d], m1 = Ja[3 * d + 1], m2 = Ja[3 * d + 2];
+#pragma unroll
+        for (int l = 0; l <= N; ++l) {
+          const int id = base + l * stride;
+          const RF dh = (RF)DH[l + n * me];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[1 + c] = acc[1 + c] + dh * (RF)sF[c * np + id];
+          if (l == me) continue;
+          const RF rs = rho + (RF)sR[id];
+          const RF us = uu + (RF)sUu[id], vs = vv + (RF)sV[id], ws = ww + (RF)sW[id];
+          const RF ps = pp + (RF)sPp[id], Hs = HH + (RF)sH[id];
+          const RF a0 = RF(0.5) * (m0 + (RF)sM[id]), a1 = RF(0.5) * (m1 + (RF)sM[np + id]),
+                   a2 = RF(0.5) * (m2 + (RF)sM[2 * np + id]);
+          const RF mv = a0 * us + a1 * vs + a2 * ws;
+          const RF qq = RF(0.25) * rs * mv;
+          const RF dv = (RF)DV[l + n * me];
+          acc[0] = acc[0] + dv * (RF(0.5) * rs * mv);
+          acc[1] = acc[1] + dv * (qq * us + a0 * ps);
+          acc[2] = acc[2] + dv * (qq * vs + a1 * ps);
+          acc[3] = acc[3] + dv * (qq * ws + a2 * ps);
+          acc[4] = acc[4] + dv * (qq * Hs);
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    // weak form: per direction stage the contravariant flux Ja^d.(f_euler + f_visc) (5 comps)
+    const double* DH = sB + B.D_Hat_T();
+    RF fe[5], ge[5], he[5];
+    if (act) {
+      const RF Ep = (RF)(Uown[4] + q.p);
+      const RF m1 = (RF)Uown[1], m2 = (RF)Uown[2], m3 = (RF)Uown[3];
+      const RF uu = (RF)q.u, vv = (RF)q.v, ww = (RF)q.w, pp = (RF)q.p;
+      fe[0] = m1; fe[1] = m1 * uu + pp; fe[2] = m1 * vv; fe[3] = m1 * ww; fe[4] = Ep * uu;
+      ge[0] = m2; ge[1] = fe[2]; ge[2] = m2 * vv + pp; ge[3] = m2 * ww; ge[4] = Ep * vv;
+      he[0] = m3; he[1] = fe[3]; he[2] = ge[3]; he[3] = m3 * ww + pp; he[4] = Ep * ww;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        fe[1 + c] = fe[1 + c] + fvx[c];
+        ge[1 + c] = ge[1 + c] + fvy[c];
+        he[1 + c] = he[1 + c] + fvz[c];
+      }
+    }
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+      if (act) {
+        const RF* Jd = Ja + 3 * d;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sA[v * np + t] = (double)(fe[v] * Jd[0] + ge[v] * Jd[1] + he[v] * Jd[2]);
+      }
+      __syncthreads();
+      if (act) {
+        const int me = d == 0 ? i : (d == 1 ? j : k);
+        const int stride = d == 0 ? 1 : (d == 1 ? n : n * n);
+        const int base = t - me * stride;
+#pragma unroll
+        for (int l = 0; l <= N; ++l) {
+          const RF dh = (RF)DH[l + n * me];
+          const int id = base + l * stride;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) acc[v] = acc[v] + dh * (RF)sA[v * np + id];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (!act) return;  // no barrier follows
+  // surface integral: master sides +Flux, slave sides -Flux (surfint.t90:21-239, 471-616)
+#pragma unroll 1
+  for (int ls = 1; ls <= 6; ++ls) {
+    int a, b, l;
+    face_coords(ls, i, j, k, a, b, l);
+    if (GL && l != (is_minus(ls) ? 0 : N)) continue;
+    const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
+    const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
+    const int* pq2 = m.v2s2 + 2 * ((a + n * b) + nf * (flip + 5 * (ls - 1)));
+    const size_t fp = (size_t)(pq2[0] + n * pq2[1]) + (size_t)nf * side;
+    const double lh = GL ? (is_minus(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N])
+                         : (is_minus(ls) ? sB[B.L_HatMinus() + l] : sB[B.L_HatPlus() + l]);
+    const RF sl = (RF)(flip == 0 ? lh : -lh);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) acc[v] = acc[v] + (RF)Flux[5 * fp + v] * sl;
+  }
+  const double sj = m.sJ[gid];
+  double ut[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) ut[v] = (-(double)acc[v]) * sj;
+  const size_t o = (size_t)5 * gid;
+  if (st.mode == 0) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Ut[o + v] = ut[v];
+  } else {
+    // VCopy / VAXPBY of TimeStepByLSERKW2 (vector.f90:74-125)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double r = (st.mode == 1) ? ut[v] : st.R[o + v] * st.minusA + ut[v];
+      st.R[o + v] = r;
+      st.U[o + v] = Uown[v] + r * st.b_dt;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// CALCTIMESTEP (calctimestep.f90:89-227): per element the maxima of the convective and viscous
+// eigenvalue sums, then dtElem; a second kernel reduces over elements.
+template <int N>
+__global__ __launch_bounds__(Dim<N>::NT) void k_dt_elem(const double* __restrict__ U, DevMesh m, Gas gas, double CFL,
+                                                      double DFL, double* __restrict__ dtConv,
+                                                      double* __restrict__ dtVisc, int* __restrict__ err) {
+  constexpr int np = Dim<N>::np, NT = Dim<N>::NT;
+  __shared__ double red[6][NT / 64];
+  __shared__ int nanflag[NT / 64];
+  const int e = blockIdx.x, t = threadIdx.x;
+  double lam[6] = {0, 0, 0, 0, 0, 0};
+  int isnan_rho = 0;
+  if (t < np) {
+    const size_t gid = (size_t)np * e + t;
+    const double* c = U + 5 * gid;
+    const double srho = 1.0 / c[0];
+    const double vel[3] = {c[1] * srho, c[2] * srho, c[3] * srho};
+    const double p = gas.kappaM1 * (c[4] - 0.5 * (vel[0] * c[1] + vel[1] * c[2] + vel[2] * c[3]));
+    if (c[0] != c[0]) isnan_rho = 1;
+    const double cs = sqrt(gas.kappa * p * srho);
+    const double sj = m.sJ[gid];
+    const double vsJ[3] = {vel[0] * sj, vel[1] * sj, vel[2] * sj};
+    const double* M[3] = {m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lam[d] = fabs(M[d][0] * vsJ[0] + M[d][1] * vsJ[1] + M[d][2] * vsJ[2]) + cs * m.MetricsAdv[3 * gid + d];
+      lam[3 + d] = gas.mu0 * srho * m.MetricsVisc[3 * gid + d];
+    }
+  }
+  // wavefront max reduction; NaN propagates through the comparison chain as in MAX(..)
+  const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double x = lam[c];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double y = __shfl_down(x, off, 64);
+      x = (y > x || y != y) ? y : x;
+    }
+    if (lane == 0) red[c][wv] = x;
+  }
+  const unsigned long long any = __ballot(isnan_rho);
+  if (lane == 0) nanflag[wv] = any ? 1 : 0;
+  __syncthreads();
+  if (t == 0) {
+    double mx[6];
+    int nanr = 0;
+    for (int c = 0; c < 6; ++c) mx[c] = red[c][0];
+    nanr = nanflag[0];
+    for (int w = 1; w < NT / 64; ++w) {
+      for (int c = 0; c < 6; ++c) mx[c] = (red[c][w] > mx[c] || red[c][w] != red[c][w]) ? red[c][w] : mx[c];
+      nanr |= nanflag[w];
+    }
+    const double dc = CFL * 2. / (mx[0] + mx[1] + mx[2]);
+    const double sv = mx[3] + mx[4] + mx[5];
+    const double dv = (sv > 0.0) ? DFL * 4. / sv : 1.79769313486231571e308;
+    dtConv[e] = dc;
+    dtVisc[e] = dv;
+    err[e] = nanr ? 1 : (dc != dc ? 2 : (dv != dv ? 3 : 0));
+  }
+}
+
+// Minimum over elements of the convective and viscous steps and the error code.
+__global__ __launch_bounds__(1024) void k_dt_reduce(const double* __restrict__ dtConv, const double* __restrict__ dtVisc,
+                                                  const int* __restrict__ err, int K, double* __restrict__ out,
+                                                  int* __restrict__ errOut) {
+  __shared__ double sc[16], sv[16];
+  __shared__ int se[16];
+  const int t = threadIdx.x;
+  double c = 1.79769313486231571e308, v = 1.79769313486231571e308;
+  int ec = 0;
+  for (int x = t; x < K; x += 1024) {
+    c = fmin(c, dtConv[x]);
+    v = fmin(v, dtVisc[x]);
+    const int ex = err[x];
+    // priority to the root cause: 1 (NaN density) over 2 (convective) over 3 (viscous)
+    if (ex != 0 && (ec == 0 || ex < ec)) ec = ex;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    c = fmin(c, __shfl_down(c, off, 64));
+    v = fmin(v, __shfl_down(v, off, 64));
+    const int eo = __shfl_down(ec, off, 64);
+    if (eo != 0 && (ec == 0 || eo < ec)) ec = eo;
+  }
+  if ((t & 63) == 0) { sc[t >> 6] = c; sv[t >> 6] = v; se[t >> 6] = ec; }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < 16; ++w) {
+      c = fmin(c, sc[w]);
+      v = fmin(v, sv[w]);
+      if (se[w] != 0 && (ec == 0 || se[w] < ec)) ec = se[w];
+    }
+    out[0] = c;
+    out[1] = v;
+    errOut[0] = ec;
+  }
+}
+
+// MetricsAdv / MetricsVisc of InitCalctimestep (calctimestep.f90:42-83).
+__global__ __launch_bounds__(256) void k_metrics_adv(DevMesh m, double KappasPr_max, double* __restrict__ adv,
+                                                   double* __restrict__ visc, int nNodes) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nNodes) return;
+  const double sj = m.sJ[x];
+  const double* M[3] = {m.Mf + 3 * (size_t)x, m.Mg + 3 * (size_t)x, m.Mh + 3 * (size_t)x};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    adv[3 * (size_t)x + d] = sj * sqrt(M[d][0] * M[d][0] + M[d][1] * M[d][1] + M[d][2] * M[d][2]);
+    const double a0 = M[d][0] * sj, a1 = M[d][1] * sj, a2 = M[d][2] * sj;
+    visc[3 * (size_t)x + d] = KappasPr_max * (a0 * a0 + a1 * a1 + a2 * a2);
+  }
+}
+
+template <class RF>
+__global__ __launch_bounds__(256) void k_unpack_grad(const RF* __restrict__ G, int np, int nNodes, double* __restrict__ gx,
+                                                   double* __restrict__ gy, double* __restrict__ gz) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nNodes) return;
+  const int e = x / np, t = x - e * np;
+  const RF* Ge = G + (size_t)12 * np * e;
+  for (int v = 0; v < 4; ++v) {
+    gx[4 * (size_t)x + v] = (double)Ge[v * np + t];
+    gy[4 * (size_t)x + v] = (double)Ge[(4 + v) * np + t];
+    gz[4 * (size_t)x + v] = (double)Ge[(8 + v) * np + t];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Dispatch on the compile-time degree.
+
+template <int N, bool GL, class RF>
+static hipError_t rhs_n(const DevMesh& m, const DevWork& w, const Gas& gas, int split, int riemann,
+                        const double* U, double* Ut, const RKStage& st, hipStream_t s) {
+  const int K = m.nElems;
+  constexpr int NT = Dim<N>::NT;
+  hipLaunchKernelGGL((k_prolong_u<N, GL>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s);
+  hipLaunchKernelGGL((k_lift<N, GL, RF>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s, gas, (RF*)w.G,
+                     (RF*)w.Gf_m, (RF*)w.Gf_s);
+  const int nPts = m.nSides * Dim<N>::nf;
+  if (split)
+    hipLaunchKernelGGL((k_flux<RF, true>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
+                       (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
+  else
+    hipLaunchKernelGGL((k_flux<RF, false>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
+                       (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
+  if (split)
+    hipLaunchKernelGGL((k_volint<N, GL, true, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut, st);
+  else
+    hipLaunchKernelGGL((k_volint<N, GL, false, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut, st);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t rhs_dispatch(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, int split,
+                               int riemann, int fp32, const double* U, double* Ut, const RKStage& st, hipStream_t s) {
+  if (nodeType == 2) {
+    if (fp32) return rhs_n<N, true, float>(m, w, gas, split, riemann, U, Ut, st, s);
+    return rhs_n<N, true, double>(m, w, gas, split, riemann, U, Ut, st, s);
+  }
+  if (split) return hipErrorInvalidValue;  // Gauss split form (entropy variables) is not built
+  if (fp32) return rhs_n<N, false, float>(m, w, gas, split, riemann, U, Ut, st, s);
+  return rhs_n<N, false, double>(m, w, gas, split, riemann, U, Ut, st, s);
+}
+
+bool supported_degree(int N) { return N >= 1 && N <= 7; }
+
+hipError_t launch_rhs(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, int split, int riemann,
+                      int fp32, const double* U, double* Ut, const RKStage& st, hipStream_t s) {
+  switch (m.N) {
+    case 1: return rhs_dispatch<1>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    case 2: return rhs_dispatch<2>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    case 3: return rhs_dispatch<3>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    case 4: return rhs_dispatch<4>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    case 5: return rhs_dispatch<5>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    case 6: return rhs_dispatch<6>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    case 7: return rhs_dispatch<7>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N>
+static hipError_t dt_n(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U, double CFL, double DFL,
+                       hipStream_t s) {
+  const int K = m.nElems;
+  double* dtConv = w.dtElem;
+  double* dtVisc = w.dtElem + K;
+  hipLaunchKernelGGL((k_dt_elem<N>), dim3(K), dim3(Dim<N>::NT), 0, s, U, m, gas, CFL, DFL, dtConv, dtVisc, w.errElem);
+  hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, s, dtConv, dtVisc, w.errElem, K, w.dtOut, w.errOut);
+  return hipGetLastError();
+}
+
+hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U, double CFL,
+                                double DFL, hipStream_t s) {
+  switch (m.N) {
+    case 1: return dt_n<1>(m, w, gas, U, CFL, DFL, s);
+    case 2: return dt_n<2>(m, w, gas, U, CFL, DFL, s);
+    case 3: return dt_n<3>(m, w, gas, U, CFL, DFL, s);
+    case 4: return dt_n<4>(m, w, gas, U, CFL, DFL, s);
+    case 5: return dt_n<5>(m, w, gas, U, CFL, DFL, s);
+    case 6: return dt_n<6>(m, w, gas, U, CFL, DFL, s);
+    case 7: return dt_n<7>(m, w, gas, U, CFL, DFL, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_metrics_adv(const DevMesh& m, double KappasPr_max, double* adv, double* visc, hipStream_t s) {
+  const int n = m.N + 1;
+  const int nNodes = n * n * n * m.nElems;
+  hipLaunchKernelGGL(k_metrics_adv, dim3((nNodes + 255) / 256), dim3(256), 0, s, m, KappasPr_max, adv, visc, nNodes);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, double* gx, double* gy, double* gz,
+                                   hipStream_t s) {
+  const int n = m.N + 1, np = n * n * n;
+  const int nNodes = np * m.nElems;
+  if (fp32)
+    hipLaunchKernelGGL(k_unpack_grad<float>, dim3((nNodes + 255) / 256), dim3(256), 0, s, (const float*)G, np, nNodes,
+                       gx, gy, gz);
+  else
+    hipLaunchKernelGGL(k_unpack_grad<double>, dim3((nNodes + 255) / 256), dim3(256), 0, s, (const double*)G, np, nNodes,
+                       gx, gy, gz);
+  return hipGetLastError();
+}
+
+}  // namespace fxg

@@ -1,0 +1,19 @@
+// Mesh metrics on the GPU: CalcMetrics/CalcSurfMetrics of FLEXI (src/mesh/metrics.f90:161-787),
+// curl form, non-conforming trees excluded.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "fxg_kernels.h"
+
+namespace fxg {
+
+// NodeCoords(3,0:NGeo,0:NGeo,0:NGeo,nElems) on the device -> Metrics_{f,g,h}Tilde, sJ, Elem_xGP
+// and, for master sides, NormVec/TangVec1/TangVec2/SurfElem. `m` supplies ElemToSide and S2V2.
+// Returns nonzero and sets `err` on failure.
+int compute_metrics_gpu(int N, int nodeType, int NGeo, int nElems, const double* NodeCoords, const DevMesh& m,
+                        double* Mf, double* Mg, double* Mh, double* sJ, double* xGP, double* NormVec,
+                        double* TangVec1, double* TangVec2, double* SurfElem, hipStream_t s, std::string& err);
+
+}  // namespace fxg

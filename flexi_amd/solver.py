@@ -1,0 +1,171 @@
+"""Host-side mirror of FLEXI's hot-path call surface on top of libfxg.
+
+``DGSolver`` keeps the FLEXI names and meanings:
+
+* ``DGTimeDerivative_weakForm(t)``  -> src/dg/dg.f90:221 (Ut from U, on the device),
+* ``TimeStepByLSERKW2(t)``          -> src/timedisc/timestep.f90:82 (uses ``self.dt``),
+* ``CalcTimeStep()``                -> src/equations/navierstokes/calctimestep.f90:89
+  (returns ``(dt, errType)``; a nonzero errType raises, as FLEXI aborts),
+
+with the state in FLEXI's layout ``U(5,0:N,0:N,0:N,nElems)`` (numpy, Fortran order) on the host.
+Device-resident use (torch tensors) goes through ``rhs_dev``/``step_dev``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .mesh import Mesh
+from .timedisc import CARPENTER_RK4_5, LSERK, fill_cfl_dfl
+
+NODE_TYPES = {"gauss": 1, "gauss-lobatto": 2, 1: 1, 2: 2}
+RIEMANN = {"lf": 1, "roe": 3, "roeentropyfix": 33, 1: 1, 3: 3, 33: 33}
+SPLITDG = {None: 0, "none": 0, "pi": 4, 0: 0, 4: 4}
+PRECISION = {"fp64": 0, "fp32-bulk": 1, "fp32": 1, 0: 0, 1: 1}
+
+
+def _f(a, dtype=np.float64):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class DGSolver:
+    """One device context of the DG operator and LSERK stepper on a periodic hexahedral mesh."""
+
+    def __init__(self, mesh: Mesh, N: int, node_type="gauss-lobatto", split_dg="pi", riemann="roe",
+                 kappa=1.4, R=71.42857, Pr=0.72, mu0=6.25e-4, precision="fp64", metrics: Optional[dict] = None,
+                 scheme: LSERK = CARPENTER_RK4_5, CFLScale=0.9, DFLScale=0.9):
+        L = _lib.lib()
+        self.mesh = mesh
+        self.N = N
+        self.node_type = NODE_TYPES[node_type]
+        self.split = SPLITDG[split_dg]
+        self.riemann = RIEMANN[riemann.lower() if isinstance(riemann, str) else riemann]
+        self.precision = PRECISION[precision]
+        self.kappa, self.R, self.Pr, self.mu0 = kappa, R, Pr, mu0
+        self.scheme = scheme
+        self.CFL, self.DFL = fill_cfl_dfl(N, self.node_type, CFLScale, DFLScale, scheme)
+        self.params = _lib.FxgParams(N, self.node_type, self.split, self.riemann, kappa, R, Pr, mu0, 0, 0,
+                                     self.precision)
+        self.ctx = C.c_void_p()
+        s2e, e2s = _f(mesh.SideToElem, np.int32), _f(mesh.ElemToSide, np.int32)
+        if metrics is None:
+            nc = _f(mesh.NodeCoords)
+            _lib.check(L.fxg_create_from_nodes(C.byref(self.params), mesh.nElems, mesh.nSides, _ptr(s2e), _ptr(e2s),
+                                               mesh.NGeo, _ptr(nc), C.byref(self.ctx)))
+        else:
+            keep = {k: _f(metrics[k]) for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ",
+                                               "NormVec", "TangVec1", "TangVec2", "SurfElem")}
+            m = _lib.FxgMesh(mesh.nElems, mesh.nSides, 1, mesh.nSides, _ptr(s2e), _ptr(e2s),
+                             *[_ptr(keep[k]) for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ",
+                                                       "NormVec", "TangVec1", "TangVec2", "SurfElem")])
+            _lib.check(L.fxg_create(C.byref(self.params), C.byref(m), C.byref(self.ctx)))
+        n = N + 1
+        self.shape = (5, n, n, n, mesh.nElems)
+        self.dt = 0.0
+        self.t = 0.0
+        self._RKA = (C.c_double * max(1, scheme.nStages - 1))(*scheme.RKA)
+        self._RKb = (C.c_double * scheme.nStages)(*scheme.RKb)
+        self._RKc = (C.c_double * max(1, scheme.nStages - 1))(*scheme.RKc)
+
+    # -- lifetime ------------------------------------------------------------------------------
+    def close(self) -> None:
+        if self.ctx:
+            _lib.lib().fxg_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state transfer (FLEXI layout) ----------------------------------------------------------
+    def set_state(self, U: np.ndarray) -> None:
+        U = _f(U)
+        if U.shape != self.shape:
+            raise ValueError(f"U must have shape {self.shape}")
+        _lib.check(_lib.lib().fxg_set_state(self.ctx, _ptr(U)))
+
+    def get_state(self) -> np.ndarray:
+        U = np.zeros(self.shape, order="F")
+        _lib.check(_lib.lib().fxg_get_state(self.ctx, _ptr(U)))
+        return U
+
+    def get_ut(self) -> np.ndarray:
+        Ut = np.zeros(self.shape, order="F")
+        _lib.check(_lib.lib().fxg_get_ut(self.ctx, _ptr(Ut)))
+        return Ut
+
+    def gradients(self):
+        n = self.N + 1
+        g = [np.zeros((4, n, n, n, self.mesh.nElems), order="F") for _ in range(3)]
+        _lib.check(_lib.lib().fxg_get_gradients(self.ctx, *[_ptr(x) for x in g]))
+        return g
+
+    def metrics(self) -> dict:
+        n, K, S = self.N + 1, self.mesh.nElems, self.mesh.nSides
+        out = dict(Metrics_fTilde=np.zeros((3, n, n, n, K), order="F"), Metrics_gTilde=np.zeros((3, n, n, n, K), order="F"),
+                   Metrics_hTilde=np.zeros((3, n, n, n, K), order="F"), sJ=np.zeros((n, n, n, K), order="F"),
+                   NormVec=np.zeros((3, n, n, S), order="F"), TangVec1=np.zeros((3, n, n, S), order="F"),
+                   TangVec2=np.zeros((3, n, n, S), order="F"), SurfElem=np.zeros((n, n, S), order="F"),
+                   Elem_xGP=np.zeros((3, n, n, n, K), order="F"))
+        _lib.check(_lib.lib().fxg_get_metrics(self.ctx, *[_ptr(v) for v in out.values()]))
+        return out
+
+    # -- FLEXI call surface ---------------------------------------------------------------------
+    def DGTimeDerivative_weakForm(self, t: float) -> None:
+        _lib.check(_lib.lib().fxg_dg_time_derivative(self.ctx, float(t)))
+
+    def TimeStepByLSERKW2(self, t: float) -> None:
+        _lib.check(_lib.lib().fxg_timestep_lserkw2(self.ctx, float(t), float(self.dt), self.scheme.nStages,
+                                                   self._RKA, self._RKb, self._RKc))
+
+    def CalcTimeStep(self, with_elements: bool = False):
+        dt, err = C.c_double(), C.c_int()
+        dtElem = np.zeros(self.mesh.nElems) if with_elements else None
+        rc = _lib.lib().fxg_calc_timestep(self.ctx, self.CFL, self.DFL, C.byref(dt), C.byref(err),
+                                          None if dtElem is None else _ptr(dtElem))
+        if rc != 0 and err.value == 0:
+            _lib.check(rc)
+        if with_elements:
+            return dt.value, err.value, dtElem
+        return dt.value, err.value
+
+    # -- convenience ----------------------------------------------------------------------------
+    def rhs(self, U: np.ndarray, t: float = 0.0) -> np.ndarray:
+        self.set_state(U)
+        self.DGTimeDerivative_weakForm(t)
+        return self.get_ut()
+
+    def run(self, n_steps: int, t: float = 0.0, fixed_dt: Optional[float] = None) -> float:
+        """UpdateTimeStep + TimeStepByLSERKW2 loop of TimeDisc (src/timedisc/timedisc.f90:190-212)."""
+        for _ in range(n_steps):
+            if fixed_dt is None:
+                dt, err = self.CalcTimeStep()
+                if err != 0:
+                    raise _lib.FxgError(f"CALCTIMESTEP errType={err} at t={t}")
+                self.dt = dt
+            else:
+                self.dt = fixed_dt
+            self.TimeStepByLSERKW2(t)
+            t += self.dt
+        self.t = t
+        return t
+
+    # -- device-pointer forms -------------------------------------------------------------------
+    def rhs_dev(self, U_ptr: int, Ut_ptr: int, t: float = 0.0, stream: int = 0) -> None:
+        _lib.check(_lib.lib().fxg_dg_time_derivative_dev(self.ctx, float(t), U_ptr, Ut_ptr, stream or None))
+
+    def step_dev(self, U_ptr: int, R_ptr: int, dt: float, t: float = 0.0, stream: int = 0) -> None:
+        _lib.check(_lib.lib().fxg_timestep_lserkw2_dev(self.ctx, float(t), float(dt), self.scheme.nStages, self._RKA,
+                                                       self._RKb, self._RKc, U_ptr, R_ptr, stream or None))
+
+    def sync(self) -> None:
+        _lib.check(_lib.lib().fxg_sync(self.ctx))

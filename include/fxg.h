@@ -1,0 +1,142 @@
+/*
+ * fxg — MI355X (gfx950) DGSEM right-hand side and low-storage Runge-Kutta stepper for the
+ * compressible Navier-Stokes equations, with the call surface of FLEXI's hot path.
+ *
+ * Plain C ABI: no C++ or torch types, plain pointers and sizes. Every array crossing this
+ * boundary uses FLEXI's own Fortran layout (column-major, variable index fastest, ids 1-based),
+ * so a FLEXI host passes its module arrays unchanged through ISO_C_BINDING.
+ *
+ * Error convention: every function returns 0 on success and a nonzero code otherwise; the
+ * message is available from fxg_last_error(). FLEXI aborts through Abort/CollectiveStop
+ * (src/globals/globals.f90:129-220); the Fortran shim maps a nonzero code onto that call.
+ *
+ * Threading: a context is bound to the HIP device that was current at creation and is not
+ * thread-safe; FLEXI calls the path from a single host thread per rank.
+ */
+#ifndef FXG_H
+#define FXG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Node types as FLEXI's PP_NodeType (src/CMakeLists.txt): 1 Gauss, 2 Gauss-Lobatto. */
+#define FXG_NODE_GAUSS 1
+#define FXG_NODE_GAUSS_LOBATTO 2
+/* Riemann solvers, numbered as src/equations/navierstokes/riemann.f90:44-57. */
+#define FXG_RIEMANN_LF 1
+#define FXG_RIEMANN_ROE 3
+#define FXG_RIEMANN_ROEENTROPYFIX 33
+/* Split-form fluxes, numbered as src/equations/navierstokes/splitflux.f90:63-69. */
+#define FXG_SPLITDG_NONE 0
+#define FXG_SPLITDG_PI 4
+/* Arithmetic of the flux, lifting and integral kernels. U, the RK register, the metric terms
+ * and the Jacobian stay FP64 in both modes. */
+#define FXG_PREC_FP64 0
+#define FXG_PREC_FP32_BULK 1
+
+/* Run-time parameters. Names and meanings follow FLEXI's parameter file keys:
+ * N, NodeType (compile-time PP_NodeType in FLEXI), SplitDG (src/equations/navierstokes/
+ * splitflux.f90:95-101), Riemann (riemann.f90:97-113), kappa/R/Pr/mu0 (idealgas/eos.f90:79-90),
+ * doWeakLifting/doConservativeLifting (src/dg/lifting/lifting.f90:72-76). */
+typedef struct {
+  int N;
+  int nodeType;
+  int splitDG;               /* FXG_SPLITDG_NONE (weak form) or FXG_SPLITDG_PI */
+  int riemann;               /* FXG_RIEMANN_* */
+  double kappa, R, Pr, mu0;  /* constant viscosity (PP_VISC=0) */
+  int doWeakLifting;         /* only 0 (strong BR1) is supported */
+  int doConservativeLifting; /* only 0 (non-conservative volume integral) is supported */
+  int precision;             /* FXG_PREC_* */
+} fxg_params;
+
+/* Mesh and metric arrays as FLEXI holds them after InitMesh (src/mesh/mesh_vars.f90:26-209):
+ *   SideToElem(5,nSides)  ElemToSide(2,6,nElems)              (1-based element and side ids)
+ *   Metrics_fTilde/gTilde/hTilde(3,0:N,0:N,0:N,nElems)  sJ(0:N,0:N,0:N,nElems)
+ *   NormVec/TangVec1/TangVec2(3,0:N,0:N,nSides)  SurfElem(0:N,0:N,nSides)
+ * Only conforming inner sides are supported (firstInnerSide = 1, lastInnerSide = nSides;
+ * a periodic mesh), i.e. nBCSides = 0, no mortars, a single rank. The library copies the
+ * arrays; the caller keeps ownership. */
+typedef struct {
+  int nElems;
+  int nSides;
+  int firstInnerSide;
+  int lastInnerSide;
+  const int* SideToElem;
+  const int* ElemToSide;
+  const double* Metrics_fTilde;
+  const double* Metrics_gTilde;
+  const double* Metrics_hTilde;
+  const double* sJ;
+  const double* NormVec;
+  const double* TangVec1;
+  const double* TangVec2;
+  const double* SurfElem;
+} fxg_mesh;
+
+typedef struct fxg_ctx fxg_ctx;
+
+/* Message of the last failed call on this thread. */
+const char* fxg_last_error(void);
+/* Library version string. */
+const char* fxg_version(void);
+
+/* Set-up: replaces InitDG/InitLifting/InitTimeDisc allocations (src/dg/dg.f90:45-111,
+ * src/dg/lifting/lifting.f90:106-185, src/timedisc/timedisc_func.f90:68-128) with device
+ * allocations. The DG basis (D, D_Hat, DVolSurf, L_Hat) is built from N and nodeType as
+ * InitDGBasis does (src/dg/dg.f90:132-209). */
+int fxg_create(const fxg_params* params, const fxg_mesh* mesh, fxg_ctx** ctx);
+/* Set-up from geometry: NodeCoords(3,0:NGeo,0:NGeo,0:NGeo,nElems) on equidistant nodes; the
+ * metrics of CalcMetrics/CalcSurfMetrics (src/mesh/metrics.f90:161-787, curl form) are computed
+ * on the GPU. SideToElem/ElemToSide as in fxg_mesh. */
+int fxg_create_from_nodes(const fxg_params* params, int nElems, int nSides,
+                          const int* SideToElem, const int* ElemToSide, int NGeo,
+                          const double* NodeCoords, fxg_ctx** ctx);
+int fxg_destroy(fxg_ctx* ctx);
+
+/* Host <-> device state transfer in FLEXI layout U(5,0:N,0:N,0:N,nElems). */
+int fxg_set_state(fxg_ctx* ctx, const double* U);
+int fxg_get_state(fxg_ctx* ctx, double* U);
+int fxg_get_ut(fxg_ctx* ctx, double* Ut);
+/* Lifted gradients of the last right-hand side, gradU{x,y,z}(4,0:N,0:N,0:N,nElems) for the
+ * lifting variables (u,v,w,T) (src/dg/lifting/lifting_vars.f90; PP_OPTLIFT=1 set, eos.h:114). */
+int fxg_get_gradients(fxg_ctx* ctx, double* gx, double* gy, double* gz);
+/* Geometry as used on the device (for checking fxg_create_from_nodes): same layouts as fxg_mesh
+ * plus Elem_xGP(3,0:N,0:N,0:N,nElems). Any pointer may be NULL. */
+int fxg_get_metrics(fxg_ctx* ctx, double* Mf, double* Mg, double* Mh, double* sJ,
+                    double* NormVec, double* TangVec1, double* TangVec2, double* SurfElem,
+                    double* Elem_xGP);
+
+/* DGTimeDerivative_weakForm(t) (src/dg/dg.f90:221-678): Ut = RHS(U) on the device state. */
+int fxg_dg_time_derivative(fxg_ctx* ctx, double t);
+/* TimeStepByLSERKW2(t) (src/timedisc/timestep.f90:82-152) for nStages stages with FLEXI's
+ * coefficient arrays RKA(2:nStages), RKb(1:nStages), RKc(2:nStages) (timedisc_vars.f90:109-214),
+ * passed as RKA[0..nStages-2], RKb[0..nStages-1], RKc[0..nStages-2]. U is advanced in place. */
+int fxg_timestep_lserkw2(fxg_ctx* ctx, double t, double dt, int nStages, const double* RKA,
+                         const double* RKb, const double* RKc);
+/* CALCTIMESTEP(errType) (src/equations/navierstokes/calctimestep.f90:89-227). CFLScale and
+ * DFLScale are the values after FillCFL_DFL (src/timedisc/timedisc_func.f90:463-527).
+ * errType: 0 ok, 1 NaN density, 2 NaN convective dt, 3 NaN viscous dt. dtElem may be NULL. */
+int fxg_calc_timestep(fxg_ctx* ctx, double CFLScale, double DFLScale, double* dt, int* errType,
+                      double* dtElem);
+
+/* Device-pointer forms for hosts that keep the state in their own device buffers (e.g. torch
+ * tensors): the arrays have the FLEXI layouts above and live on the context's device; `stream`
+ * is a hipStream_t (NULL = the context's stream). The call is asynchronous on that stream. */
+int fxg_dg_time_derivative_dev(fxg_ctx* ctx, double t, const double* U_dev, double* Ut_dev,
+                               void* stream);
+int fxg_timestep_lserkw2_dev(fxg_ctx* ctx, double t, double dt, int nStages, const double* RKA,
+                             const double* RKb, const double* RKc, double* U_dev, double* R_dev,
+                             void* stream);
+/* Synchronise the context's stream. */
+int fxg_sync(fxg_ctx* ctx);
+
+/* Sizes of the context (for hosts that allocate the device-pointer arrays themselves). */
+int fxg_sizes(fxg_ctx* ctx, int* N, int* nElems, int* nSides);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FXG_H */
