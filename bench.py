@@ -1,0 +1,167 @@
+"""Benchmark: DOF-stage updates per second of the DGSEM Navier-Stokes step on one MI355X.
+
+Workload (one "step" = FLEXI's time-loop body: CALCTIMESTEP + TimeStepByLSERKW2 with the
+5-stage Carpenter RK4-5, i.e. 5 right-hand sides and 5 register updates):
+Taylor-Green vortex (Ma 0.1, Re 1600) on a periodic 32^3 hexahedral mesh, N = 7 Gauss-Lobatto
+nodes (16,777,216 DOF), split form with the Pirozzoli flux, Roe Riemann solver, BR1 lifting.
+DOF = one volume node (FLEXI's convention); value = DOF * stages * steps / wall time.
+
+Prints one JSON line. Options: --gpus (only 1 exists on this platform), --steps, --warmup,
+--elems (per direction), --no-cpu (skip the CPU baseline), --no-fp32.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
+# Algorithmic HBM bytes per DOF of each kernel (FP64, N=7, Gauss-Lobatto); the derivation from
+# the per-DOF-stage byte model (432 B) is tabulated in DESIGN.md ("Byte model").
+ALGO_BYTES_PER_DOF = {"prolong_u": 60.0, "lift": 372.0, "flux": 147.0, "volint": 366.0}
+B_STAGE = 432.0
+
+
+def make_solver(nel, precision):
+    from flexi_amd.mesh import periodic_box
+    from flexi_amd.solver import DGSolver
+    mesh = periodic_box(nel, nel, nel)
+    s = DGSolver(mesh, 7, node_type="gauss-lobatto", split_dg="pi", riemann="roe", precision=precision,
+                 CFLScale=0.8, DFLScale=0.8)
+    return mesh, s
+
+
+def run_gpu(nel, precision, steps, warmup, profile):
+    mesh, s = make_solver(nel, precision)
+    U0 = s_tgv(s)
+    s.set_state(U0)
+    del U0
+    t = s.run(warmup)
+    s.sync()
+    if profile:
+        s.profile(True)
+    t0 = time.perf_counter()
+    t = s.run(steps, t=t)
+    s.sync()
+    wall = time.perf_counter() - t0
+    kt = s.kernel_times() if profile else None
+    # RHS-only rate (DGTimeDerivative_weakForm alone), 50 evaluations
+    s.profile(False)
+    s.sync()
+    r0 = time.perf_counter()
+    for _ in range(50):
+        s.DGTimeDerivative_weakForm(t)
+    s.sync()
+    rhs_wall = (time.perf_counter() - r0) / 50
+    dof = mesh.nElems * 8 ** 3
+    s.close()
+    return dict(wall=wall, kernels=kt, dof=dof, rhs_s=rhs_wall, t=t)
+
+
+def s_tgv(s):
+    from flexi_amd import testcase
+    xgp = s.metrics()["Elem_xGP"]
+    return testcase.tgv(xgp)
+
+
+def cpu_baseline(nel):
+    """The CPU oracle (a C++/OpenMP restatement of the reference) on one LSERK step of the same
+    workload, on the host cores of this machine."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from flexi_amd.mesh import periodic_box
+    from flexi_amd import testcase
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc.set_threads(threads)
+    mesh = periodic_box(nel, nel, nel)
+    prm = orc.params(7, orc.NODE_GL, 1, orc.RIEMANN_ROE)
+    m = orc.calc_metrics(7, orc.NODE_GL, mesh)
+    S = orc.Solver(prm, mesh, m)
+    U = testcase.tgv(m["Elem_xGP"])
+    from flexi_amd.timedisc import fill_cfl_dfl
+    cfl, dfl = fill_cfl_dfl(7, 2, 0.8, 0.8)
+    R = orc.F(U.shape)
+    t0 = time.perf_counter()
+    dt, err, _ = S.calc_timestep(U, cfl, dfl)
+    S.step(U, 0.0, dt, R)
+    wall = time.perf_counter() - t0
+    dof = mesh.nElems * 8 ** 3
+    return dict(value=dof * 5 / wall, unit="DOF-stage/s", cores=threads, kind="port",
+                sample=f"1 CarpenterRK4-5 step (5 stages + CALCTIMESTEP) of the {nel}^3 N=7 workload, "
+                       f"{wall:.1f} s wall, OpenMP over elements")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--elems", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build targets one MI355X per run "
+              f"(the multi-GPU face exchange is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    print(f"[bench] fp64 {args.elems}^3 N=7: {args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr,
+          flush=True)
+    r64 = run_gpu(args.elems, "fp64", args.steps, args.warmup, profile=True)
+    stages = 5
+    value = r64["dof"] * stages * args.steps / r64["wall"]
+    ms_per_step = 1e3 * r64["wall"] / args.steps
+    kern = {k: {"avg_ms": (v[0] / v[1] if v[1] else None), "launches": v[1]} for k, v in r64["kernels"].items()}
+    # dominant kernel and its roofline (algorithmic bytes / measured average launch time)
+    rhs_k = {k: v for k, v in r64["kernels"].items() if k in ALGO_BYTES_PER_DOF and v[1]}
+    dom = max(rhs_k, key=lambda k: rhs_k[k][0])
+    avg_s = rhs_k[dom][0] / rhs_k[dom][1] * 1e-3
+    achieved = ALGO_BYTES_PER_DOF[dom] * r64["dof"] / avg_s / 1e9
+    stage_s = sum(v[0] for k, v in rhs_k.items()) / (args.steps * stages) * 1e-3
+    stage_achieved = B_STAGE * r64["dof"] / stage_s / 1e9
+    out = {
+        "metric": "DOF-stage updates/s at N=7 (FP64 vs FP32-bulk) + adjoint/forward time ratio",
+        "value": value,
+        "unit": "DOF-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green vortex initial state, Ma 0.1)",
+        "config": {"workload": f"TGV {args.elems}^3 hex, N=7 Gauss-Lobatto, split-form PI, Roe, BR1, "
+                               f"CarpenterRK4-5 with CALCTIMESTEP each step",
+                   "dof": r64["dof"], "stages_per_step": stages, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_dof": ALGO_BYTES_PER_DOF[dom], "avg_launch_ms": avg_s * 1e3},
+        "roofline_stage": {"bytes_per_dof_stage": B_STAGE, "stage_ms": stage_s * 1e3, "achieved": stage_achieved,
+                           "unit": "GB/s", "frac": stage_achieved / HBM_PEAK_GBS,
+                           "dof_stage_per_s_at_peak": HBM_PEAK_GBS * 1e9 / B_STAGE},
+        "kernels": kern,
+        "rhs_only": {"value": r64["dof"] / r64["rhs_s"], "unit": "DOF/s (one RHS)", "ms": r64["rhs_s"] * 1e3},
+    }
+    if not args.no_fp32:
+        print("[bench] fp32-bulk", file=sys.stderr, flush=True)
+        r32 = run_gpu(args.elems, "fp32-bulk", args.steps, args.warmup, profile=False)
+        out["fp32_bulk"] = {"value": r32["dof"] * stages * args.steps / r32["wall"], "unit": "DOF-stage/s",
+                            "ms_per_step": 1e3 * r32["wall"] / args.steps}
+    out["adjoint_forward_ratio"] = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline (oracle, one step)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.elems)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

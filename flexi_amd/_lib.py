@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
     L.fxg_timestep_lserkw2_dev.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, dp, dp, vp, vp, vp]
     L.fxg_sync.argtypes = [vp]
     L.fxg_sizes.argtypes = [vp, ip, ip, ip]
+    L.fxg_profile_enable.argtypes = [vp, C.c_int]
+    L.fxg_profile_read.argtypes = [vp, C.c_int, dp, ip]
     _lib = L
     return L
 
@@ -65,7 +67,9 @@ def lib() -> C.CDLL:
 EXPORTS = ["fxg_last_error", "fxg_version", "fxg_create", "fxg_create_from_nodes", "fxg_destroy", "fxg_set_state",
            "fxg_get_state", "fxg_get_ut", "fxg_get_gradients", "fxg_get_metrics", "fxg_dg_time_derivative",
            "fxg_timestep_lserkw2", "fxg_calc_timestep", "fxg_dg_time_derivative_dev", "fxg_timestep_lserkw2_dev",
-           "fxg_sync", "fxg_sizes"]
+           "fxg_sync", "fxg_sizes", "fxg_profile_enable", "fxg_profile_read"]
+
+KERNEL_NAMES = ["prolong_u", "lift", "flux", "volint", "dt_elem", "dt_reduce"]
 
 
 def check(rc: int) -> None:

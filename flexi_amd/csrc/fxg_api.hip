@@ -98,6 +98,30 @@ void build_maps(int N, std::vector<int>& s2v2, std::vector<int>& v2s2) {
         }
 }
 
+// Event pairs around every kernel launch, read back after the timed region.
+struct EventTimer : fxg::KernelTimer {
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, size_t>> marks;  // (kernel id, event index)
+  hipError_t err = hipSuccess;
+  void mark(int id, bool start, hipStream_t s) override {
+    (void)start;
+    hipEvent_t ev;
+    if (marks.size() < pool.size()) {
+      ev = pool[marks.size()];
+    } else {
+      hipError_t e = hipEventCreate(&ev);
+      if (e != hipSuccess) { err = e; return; }
+      pool.push_back(ev);
+    }
+    hipError_t e = hipEventRecord(ev, s);
+    if (e != hipSuccess) err = e;
+    marks.emplace_back(id, marks.size());
+  }
+  ~EventTimer() override {
+    for (auto ev : pool) (void)hipEventDestroy(ev);
+  }
+};
+
 template <class T>
 int dev_alloc(T** p, size_t count) {
   if (count == 0) count = 1;
@@ -124,6 +148,7 @@ struct fxg_ctx {
   double *dtHost = nullptr;
   int* errHost = nullptr;
   std::vector<void*> owned;
+  EventTimer* timer = nullptr;  // non-null while kernel timing is enabled
 };
 
 namespace {
@@ -237,6 +262,7 @@ int finish_setup(fxg_ctx* c) {
 
 void ctx_free(fxg_ctx* c) {
   if (!c) return;
+  delete c->timer;
   for (void* ptr : c->owned) (void)hipFree(ptr);
   if (c->dtHost) (void)hipHostFree(c->dtHost);
   if (c->errHost) (void)hipHostFree(c->errHost);
@@ -246,7 +272,7 @@ void ctx_free(fxg_ctx* c) {
 
 int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStream_t s) {
   HIPCHECK(fxg::launch_rhs(c->m, c->w, c->gas, c->p.nodeType, c->p.splitDG == FXG_SPLITDG_PI, c->p.riemann,
-                           c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st, s));
+                           c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st, s, c->timer));
   return 0;
 }
 
@@ -424,9 +450,37 @@ int fxg_sync(fxg_ctx* c) {
   return 0;
 }
 
+int fxg_profile_enable(fxg_ctx* c, int enable) {
+  if (!c) return fail("ctx is NULL");
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  delete c->timer;
+  c->timer = enable ? new EventTimer : nullptr;
+  return 0;
+}
+
+int fxg_profile_read(fxg_ctx* c, int kernel, double* total_ms, int* launches) {
+  if (!c || !total_ms || !launches) return fail("NULL argument");
+  if (!c->timer) return fail("kernel timing is not enabled");
+  if (c->timer->err != hipSuccess) return fail(std::string("event timer: ") + hipGetErrorString(c->timer->err));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  int cnt = 0;
+  const auto& mk = c->timer->marks;
+  for (size_t i = 0; i + 1 < mk.size(); i += 2) {
+    if (mk[i].first != kernel) continue;
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c->timer->pool[mk[i].second], c->timer->pool[mk[i + 1].second]));
+    tot += ms;
+    ++cnt;
+  }
+  *total_ms = tot;
+  *launches = cnt;
+  return 0;
+}
+
 int fxg_calc_timestep(fxg_ctx* c, double CFL, double DFL, double* dt, int* errType, double* dtElem) {
   if (!c || !dt || !errType) return fail("NULL argument");
-  HIPCHECK(fxg::launch_calc_timestep(c->m, c->w, c->gas, c->U, CFL, DFL, c->stream));
+  HIPCHECK(fxg::launch_calc_timestep(c->m, c->w, c->gas, c->U, CFL, DFL, c->stream, c->timer));
   HIPCHECK(hipMemcpyAsync(c->dtHost, c->w.dtOut, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipMemcpyAsync(c->errHost, c->w.errOut, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if (dtElem) {

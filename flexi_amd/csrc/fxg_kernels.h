@@ -62,13 +62,22 @@ struct RKStage {
   double* R;       // LSERK register Ut_tmp
 };
 
+// Kernel ids for the optional event timer (fxg_profile_read).
+enum KernelId { K_PROLONG = 0, K_LIFT = 1, K_FLUX = 2, K_VOLINT = 3, K_DT_ELEM = 4, K_DT_REDUCE = 5, K_COUNT = 6 };
+
+// Records a HIP event before (start = true) and after each kernel launch when timing is on.
+struct KernelTimer {
+  virtual void mark(int id, bool start, hipStream_t s) = 0;
+  virtual ~KernelTimer() = default;
+};
+
 // Runs prolongation, lifting, fluxes and the volume/surface integrals for one RHS.
-// Ut may be null when stage.mode != 0.
+// Ut may be null when stage.mode != 0. `timer` may be null.
 hipError_t launch_rhs(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, int split,
                       int riemann, int fp32, const double* U, double* Ut, const RKStage& stage,
-                      hipStream_t s);
+                      hipStream_t s, KernelTimer* timer);
 hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U,
-                                double CFL, double DFL, hipStream_t s);
+                                double CFL, double DFL, hipStream_t s, KernelTimer* timer);
 hipError_t launch_metrics_adv(const DevMesh& m, double KappasPr_max, double* adv, double* visc,
                               hipStream_t s);
 // Copies a (12, np, nElems) gradient array of type float/double into three FLEXI-layout

@@ -579,77 +579,111 @@ __global__ __launch_bounds__(256) void k_unpack_grad(const RF* __restrict__ G, i
 // ---------------------------------------------------------------------------------------------
 // Dispatch on the compile-time degree.
 
+struct TimedScope {
+  KernelTimer* t;
+  int id;
+  hipStream_t s;
+  TimedScope(KernelTimer* t_, int id_, hipStream_t s_) : t(t_), id(id_), s(s_) {
+    if (t) t->mark(id, true, s);
+  }
+  ~TimedScope() {
+    if (t) t->mark(id, false, s);
+  }
+};
+
 template <int N, bool GL, class RF>
 static hipError_t rhs_n(const DevMesh& m, const DevWork& w, const Gas& gas, int split, int riemann,
-                        const double* U, double* Ut, const RKStage& st, hipStream_t s) {
+                        const double* U, double* Ut, const RKStage& st, hipStream_t s, KernelTimer* tm) {
   const int K = m.nElems;
   constexpr int NT = Dim<N>::NT;
-  hipLaunchKernelGGL((k_prolong_u<N, GL>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s);
-  hipLaunchKernelGGL((k_lift<N, GL, RF>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s, gas, (RF*)w.G,
-                     (RF*)w.Gf_m, (RF*)w.Gf_s);
+  {
+    TimedScope ts(tm, K_PROLONG, s);
+    hipLaunchKernelGGL((k_prolong_u<N, GL>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s);
+  }
+  {
+    TimedScope ts(tm, K_LIFT, s);
+    hipLaunchKernelGGL((k_lift<N, GL, RF>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s, gas, (RF*)w.G,
+                       (RF*)w.Gf_m, (RF*)w.Gf_s);
+  }
   const int nPts = m.nSides * Dim<N>::nf;
-  if (split)
-    hipLaunchKernelGGL((k_flux<RF, true>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
-                       (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
-  else
-    hipLaunchKernelGGL((k_flux<RF, false>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
-                       (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
-  if (split)
-    hipLaunchKernelGGL((k_volint<N, GL, true, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut, st);
-  else
-    hipLaunchKernelGGL((k_volint<N, GL, false, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut, st);
+  {
+    TimedScope ts(tm, K_FLUX, s);
+    if (split)
+      hipLaunchKernelGGL((k_flux<RF, true>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
+                         (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
+    else
+      hipLaunchKernelGGL((k_flux<RF, false>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
+                         (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
+  }
+  {
+    TimedScope ts(tm, K_VOLINT, s);
+    if (split)
+      hipLaunchKernelGGL((k_volint<N, GL, true, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut,
+                         st);
+    else
+      hipLaunchKernelGGL((k_volint<N, GL, false, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut,
+                         st);
+  }
   return hipGetLastError();
 }
 
 template <int N>
 static hipError_t rhs_dispatch(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, int split,
-                               int riemann, int fp32, const double* U, double* Ut, const RKStage& st, hipStream_t s) {
+                               int riemann, int fp32, const double* U, double* Ut, const RKStage& st, hipStream_t s,
+                               KernelTimer* tm) {
   if (nodeType == 2) {
-    if (fp32) return rhs_n<N, true, float>(m, w, gas, split, riemann, U, Ut, st, s);
-    return rhs_n<N, true, double>(m, w, gas, split, riemann, U, Ut, st, s);
+    if (fp32) return rhs_n<N, true, float>(m, w, gas, split, riemann, U, Ut, st, s, tm);
+    return rhs_n<N, true, double>(m, w, gas, split, riemann, U, Ut, st, s, tm);
   }
   if (split) return hipErrorInvalidValue;  // Gauss split form (entropy variables) is not built
-  if (fp32) return rhs_n<N, false, float>(m, w, gas, split, riemann, U, Ut, st, s);
-  return rhs_n<N, false, double>(m, w, gas, split, riemann, U, Ut, st, s);
+  if (fp32) return rhs_n<N, false, float>(m, w, gas, split, riemann, U, Ut, st, s, tm);
+  return rhs_n<N, false, double>(m, w, gas, split, riemann, U, Ut, st, s, tm);
 }
 
 bool supported_degree(int N) { return N >= 1 && N <= 7; }
 
 hipError_t launch_rhs(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, int split, int riemann,
-                      int fp32, const double* U, double* Ut, const RKStage& st, hipStream_t s) {
+                      int fp32, const double* U, double* Ut, const RKStage& st, hipStream_t s, KernelTimer* tm) {
   switch (m.N) {
-    case 1: return rhs_dispatch<1>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
-    case 2: return rhs_dispatch<2>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
-    case 3: return rhs_dispatch<3>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
-    case 4: return rhs_dispatch<4>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
-    case 5: return rhs_dispatch<5>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
-    case 6: return rhs_dispatch<6>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
-    case 7: return rhs_dispatch<7>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s);
+    case 1: return rhs_dispatch<1>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s, tm);
+    case 2: return rhs_dispatch<2>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s, tm);
+    case 3: return rhs_dispatch<3>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s, tm);
+    case 4: return rhs_dispatch<4>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s, tm);
+    case 5: return rhs_dispatch<5>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s, tm);
+    case 6: return rhs_dispatch<6>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s, tm);
+    case 7: return rhs_dispatch<7>(m, w, gas, nodeType, split, riemann, fp32, U, Ut, st, s, tm);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int N>
 static hipError_t dt_n(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U, double CFL, double DFL,
-                       hipStream_t s) {
+                       hipStream_t s, KernelTimer* tm) {
   const int K = m.nElems;
   double* dtConv = w.dtElem;
   double* dtVisc = w.dtElem + K;
-  hipLaunchKernelGGL((k_dt_elem<N>), dim3(K), dim3(Dim<N>::NT), 0, s, U, m, gas, CFL, DFL, dtConv, dtVisc, w.errElem);
-  hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, s, dtConv, dtVisc, w.errElem, K, w.dtOut, w.errOut);
+  {
+    TimedScope ts(tm, K_DT_ELEM, s);
+    hipLaunchKernelGGL((k_dt_elem<N>), dim3(K), dim3(Dim<N>::NT), 0, s, U, m, gas, CFL, DFL, dtConv, dtVisc,
+                       w.errElem);
+  }
+  {
+    TimedScope ts(tm, K_DT_REDUCE, s);
+    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, s, dtConv, dtVisc, w.errElem, K, w.dtOut, w.errOut);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U, double CFL,
-                                double DFL, hipStream_t s) {
+                                double DFL, hipStream_t s, KernelTimer* tm) {
   switch (m.N) {
-    case 1: return dt_n<1>(m, w, gas, U, CFL, DFL, s);
-    case 2: return dt_n<2>(m, w, gas, U, CFL, DFL, s);
-    case 3: return dt_n<3>(m, w, gas, U, CFL, DFL, s);
-    case 4: return dt_n<4>(m, w, gas, U, CFL, DFL, s);
-    case 5: return dt_n<5>(m, w, gas, U, CFL, DFL, s);
-    case 6: return dt_n<6>(m, w, gas, U, CFL, DFL, s);
-    case 7: return dt_n<7>(m, w, gas, U, CFL, DFL, s);
+    case 1: return dt_n<1>(m, w, gas, U, CFL, DFL, s, tm);
+    case 2: return dt_n<2>(m, w, gas, U, CFL, DFL, s, tm);
+    case 3: return dt_n<3>(m, w, gas, U, CFL, DFL, s, tm);
+    case 4: return dt_n<4>(m, w, gas, U, CFL, DFL, s, tm);
+    case 5: return dt_n<5>(m, w, gas, U, CFL, DFL, s, tm);
+    case 6: return dt_n<6>(m, w, gas, U, CFL, DFL, s, tm);
+    case 7: return dt_n<7>(m, w, gas, U, CFL, DFL, s, tm);
     default: return hipErrorInvalidValue;
   }
 }

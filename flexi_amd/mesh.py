@@ -13,9 +13,9 @@ all 1-based like the Fortran arrays. A periodic box has no boundary sides, so ev
 inner side (``firstInnerSide = 1``, ``lastInnerSide = nSides``; src/mesh/mesh.f90:257-268).
 
 Local side ids follow ``flexi.h``: 1 zeta-, 2 eta-, 3 xi+, 4 eta+, 5 xi-, 6 zeta+.
-The master of a side is the element that reaches it first in element order (flip 0); the
-slave's flip is found geometrically by matching the face corners through the side-to-volume
-mapping of src/mesh/mappings.f90:188-446, so it is right for any orientation.
+The master of a side is the element on its minus side (flip 0); the slave's flip is found
+geometrically by matching the face corners through the side-to-volume mapping of
+src/mesh/mappings.f90:188-446.
 """
 from __future__ import annotations
 
@@ -110,68 +110,68 @@ def periodic_box(nx: int, ny: int, nz: int,
                  deform: Optional[Callable[[np.ndarray], np.ndarray]] = None) -> Mesh:
     """Fully periodic box of ``nx*ny*nz`` hexahedra (element id = ix + nx*(iy + ny*iz) + 1).
 
-    ``deform`` maps an array of points ``(..., 3)`` to displaced points; it must be periodic
-    on the box so that the periodic faces still match (e.g. ``x + 0.1*sin x sin y sin z`` on
-    ``[-pi,pi]^3``). Each direction needs at least two elements so that the corner matching
-    that determines the flips is unambiguous.
+    Side ``3*(e-1) + d + 1`` is the face between element ``e`` and its neighbour in the positive
+    direction ``d`` (0: xi, 1: eta, 2: zeta); ``e`` is its master through its plus side, the
+    neighbour its slave through its minus side. ``deform`` maps points ``(..., 3)`` to displaced
+    points; it must be periodic on the box so that periodic faces still match. Each direction
+    needs at least two elements so that the corner matching that fixes the flips is unambiguous.
     """
     if min(nx, ny, nz) < 2:
         raise ValueError("periodic_box needs at least 2 elements per direction")
     K = nx * ny * nz
     dims = (nx, ny, nz)
-    h = [length[d] / dims[d] for d in range(3)]
+    h = np.array([length[d] / dims[d] for d in range(3)])
+    e = np.arange(K)
+    ijk = np.stack([e % nx, (e // nx) % ny, e // (nx * ny)])  # (3,K)
 
-    # Equidistant geometry nodes of every element.
+    # Equidistant geometry nodes of every element, (3,ng,ng,ng,K) in Fortran order.
     xi = np.linspace(0.0, 1.0, NGeo + 1)
-    NodeCoords = np.zeros((3, NGeo + 1, NGeo + 1, NGeo + 1, K), order="F")
-    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
-    elem_ijk = np.stack([ix.ravel(order="F"), iy.ravel(order="F"), iz.ravel(order="F")], axis=1)
-    a, b, c = np.meshgrid(xi, xi, xi, indexing="ij")
-    for e in range(K):
-        ex, ey, ez = elem_ijk[e]
-        pts = np.stack([lo[0] + (ex + a) * h[0], lo[1] + (ey + b) * h[1], lo[2] + (ez + c) * h[2]], axis=-1)
-        if deform is not None:
-            pts = deform(pts)
-        NodeCoords[:, :, :, :, e] = np.moveaxis(pts, -1, 0)
+    a = xi[:, None, None, None]
+    b = xi[None, :, None, None]
+    c = xi[None, None, :, None]
+    pts = np.stack(np.broadcast_arrays(lo[0] + (ijk[0] + a) * h[0], lo[1] + (ijk[1] + b) * h[1],
+                                       lo[2] + (ijk[2] + c) * h[2]), axis=-1)  # (ng,ng,ng,K,3)
+    if deform is not None:
+        pts = deform(pts)
+    NodeCoords = np.asfortranarray(np.moveaxis(pts, -1, 0))
 
-    def elem_id(ex, ey, ez):
-        return (ex % nx) + nx * ((ey % ny) + ny * (ez % nz))
+    # Flip of the slave (minus side) relative to the master (plus side), one per direction:
+    # match the four face corners on the integer lattice of a 2x2x2 element block.
+    plus = {0: XI_PLUS, 1: ETA_PLUS, 2: ZETA_PLUS}
+    minus = {0: XI_MINUS, 1: ETA_MINUS, 2: ZETA_MINUS}
+    flips = {}
+    for d in range(3):
+        off = np.zeros(3, dtype=int)
+        off[d] = 1
+        master = [tuple(np.array(side_to_vol(1, 0, p, q, 0, plus[d]))) for p in (0, 1) for q in (0, 1)]
+        for f in range(1, 5):
+            slave = [tuple(np.array(side_to_vol(1, 0, p, q, f, minus[d])) + off) for p in (0, 1) for q in (0, 1)]
+            if slave == master:
+                flips[d] = f
+                break
+        if d not in flips:
+            raise RuntimeError("no consistent slave orientation found")
 
-    def corner_lattice(e, ls, p, q, flip):
-        i, j, k = side_to_vol(1, 0, p, q, flip, ls)
-        ex, ey, ez = elem_ijk[e]
-        return ((ex + i) % nx, (ey + j) % ny, (ez + k) % nz)
-
+    nb = []
+    for d in range(3):
+        o = ijk.copy()
+        o[d] = (o[d] + 1) % dims[d]
+        nb.append(o[0] + nx * (o[1] + ny * o[2]))
+    S = 3 * K
+    SideToElem = np.zeros((5, S), dtype=np.int32, order="F")
     ElemToSide = np.zeros((2, 6, K), dtype=np.int32, order="F")
-    s2e = []
-    assigned = {}
-    for e in range(K):
-        ex, ey, ez = elem_ijk[e]
-        for ls in range(1, 7):
-            if (e, ls) in assigned:
-                continue
-            d, sgn = _SIDE_DIR[ls]
-            off = [0, 0, 0]
-            off[d] = sgn
-            nb = elem_id(ex + off[0], ey + off[1], ez + off[2])
-            nbls = [s for s, (dd, ss) in _SIDE_DIR.items() if dd == d and ss == -sgn][0]
-            master = [corner_lattice(e, ls, p, q, 0) for p in (0, 1) for q in (0, 1)]
-            flip = None
-            for f in range(5):
-                if [corner_lattice(nb, nbls, p, q, f) for p in (0, 1) for q in (0, 1)] == master:
-                    flip = f
-                    break
-            if flip is None or flip == 0:
-                raise RuntimeError("no consistent slave orientation found")
-            sid = len(s2e) + 1
-            s2e.append((e + 1, nb + 1, ls, nbls, flip))
-            assigned[(e, ls)] = (sid, 0)
-            assigned[(nb, nbls)] = (sid, flip)
-    for (e, ls), (sid, flip) in assigned.items():
-        ElemToSide[0, ls - 1, e] = sid
-        ElemToSide[1, ls - 1, e] = flip
-    SideToElem = np.asfortranarray(np.array(s2e, dtype=np.int32).T)
-    return Mesh(nElems=K, nSides=len(s2e), NGeo=NGeo, NodeCoords=NodeCoords, ElemToSide=ElemToSide,
+    for d in range(3):
+        sid = 3 * e + d  # 0-based
+        SideToElem[0, sid] = e + 1
+        SideToElem[1, sid] = nb[d] + 1
+        SideToElem[2, sid] = plus[d]
+        SideToElem[3, sid] = minus[d]
+        SideToElem[4, sid] = flips[d]
+        ElemToSide[0, plus[d] - 1, e] = sid + 1
+        ElemToSide[1, plus[d] - 1, e] = 0
+        ElemToSide[0, minus[d] - 1, nb[d]] = sid + 1
+        ElemToSide[1, minus[d] - 1, nb[d]] = flips[d]
+    return Mesh(nElems=K, nSides=S, NGeo=NGeo, NodeCoords=NodeCoords, ElemToSide=ElemToSide,
                 SideToElem=SideToElem, shape=dims, box_lo=tuple(lo), box_len=tuple(length))
 
 

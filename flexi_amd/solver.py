@@ -169,3 +169,16 @@ class DGSolver:
 
     def sync(self) -> None:
         _lib.check(_lib.lib().fxg_sync(self.ctx))
+
+    # -- kernel timing --------------------------------------------------------------------------
+    def profile(self, enable: bool = True) -> None:
+        _lib.check(_lib.lib().fxg_profile_enable(self.ctx, int(enable)))
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (total ms, launches)} since profile(True)."""
+        out = {}
+        for kid, name in enumerate(_lib.KERNEL_NAMES):
+            ms, cnt = C.c_double(), C.c_int()
+            _lib.check(_lib.lib().fxg_profile_read(self.ctx, kid, C.byref(ms), C.byref(cnt)))
+            out[name] = (ms.value, cnt.value)
+        return out

@@ -136,6 +136,15 @@ int fxg_sync(fxg_ctx* ctx);
 /* Sizes of the context (for hosts that allocate the device-pointer arrays themselves). */
 int fxg_sizes(fxg_ctx* ctx, int* N, int* nElems, int* nSides);
 
+/* Kernel timing with HIP events (no counterpart in the reference, whose timers are wall-clock
+ * FLEXITIME calls around the whole time loop, src/globals/globals.f90:660). While enabled, an
+ * event pair brackets every kernel launch on the launch stream; fxg_profile_read synchronises
+ * and returns the summed duration and launch count of one kernel since enabling.
+ * Kernel ids: 0 prolongation, 1 lifting, 2 side fluxes, 3 volume/surface integral (+RK update),
+ * 4 per-element time step, 5 time-step reduction. Enabling again resets the counters. */
+int fxg_profile_enable(fxg_ctx* ctx, int enable);
+int fxg_profile_read(fxg_ctx* ctx, int kernel, double* total_ms, int* launches);
+
 #ifdef __cplusplus
 }
 #endif
