@@ -1,0 +1,195 @@
+"""Pin the oracle's DG operator where FLEXI's fixtures do not reach: closed forms, invariants and
+the reference's own Taylor-Green regression trajectory (regressioncheck/checks/tgv/split)."""
+import numpy as np
+import pytest
+
+import flexi_fixtures as fx
+import oracle as orc
+from flexi_amd import testcase
+from flexi_amd.mesh import periodic_box, sine_deform
+
+PRM = dict(kappa=1.4, R=71.42857, Pr=0.72, mu0=6.25e-4)
+
+
+def euler_flux(U):
+    rho, m = U[0], U[1:4]
+    v = m / rho
+    p = 0.4 * (U[4] - 0.5 * m @ v)
+    f = np.zeros((5, 3))
+    f[0] = m
+    f[1:4] = np.outer(m, v) + p * np.eye(3)
+    f[4] = (U[4] + p) * v
+    return f, np.r_[rho, v, p, p / (rho * PRM["R"])]
+
+
+def random_state(rng):
+    prim = np.r_[rng.uniform(0.5, 2.0), rng.normal(size=3), rng.uniform(0.5, 3.0)]
+    U = np.r_[prim[0], prim[0] * prim[1:4], prim[4] / 0.4 + 0.5 * prim[0] * prim[1:4] @ prim[1:4]]
+    return np.asfortranarray(U)
+
+
+def test_split_volume_flux_consistency_factor_two():
+    # F#(u,u; Ja,Ja) = 2 Ja.(f,g,h)(u)  (splitflux.f90:21-22 factor-2 convention)
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        U = random_state(rng)
+        f, P = euler_flux(U)
+        Ja = np.asfortranarray(rng.normal(size=3))
+        out = np.zeros(5, order="F")
+        orc.lib().or_split_volume_flux_pi(U, np.asfortranarray(P), U, np.asfortranarray(P), Ja, Ja, out)
+        np.testing.assert_allclose(out, 2 * f @ Ja, rtol=1e-13, atol=1e-13)
+
+
+def test_split_volume_flux_symmetry():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        U1, U2 = random_state(rng), random_state(rng)
+        P1, P2 = (np.asfortranarray(euler_flux(u)[1]) for u in (U1, U2))
+        M1, M2 = (np.asfortranarray(rng.normal(size=3)) for _ in range(2))
+        a, b = np.zeros(5, order="F"), np.zeros(5, order="F")
+        orc.lib().or_split_volume_flux_pi(U1, P1, U2, P2, M1, M2, a)
+        orc.lib().or_split_volume_flux_pi(U2, P2, U1, P1, M2, M1, b)
+        np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-14)
+
+
+def orthonormal(n, rng):
+    n = n / np.linalg.norm(n)
+    t = rng.normal(size=3)
+    t1 = t - (t @ n) * n
+    t1 /= np.linalg.norm(t1)
+    return n, t1, np.cross(n, t1)
+
+
+@pytest.mark.parametrize("riemann", [orc.RIEMANN_LF, orc.RIEMANN_ROE, orc.RIEMANN_ROEENTROPYFIX])
+@pytest.mark.parametrize("split", [0, 1])
+def test_riemann_consistency_and_rotation_invariance(riemann, split):
+    rng = np.random.default_rng(2)
+    prm = orc.params(3, orc.NODE_GL, split, riemann, **PRM)
+    for _ in range(10):
+        U = random_state(rng)
+        f, P = euler_flux(U)
+        n, t1, t2 = orthonormal(rng.normal(size=3), rng)
+        F = np.zeros(5, order="F")
+        Pf = np.asfortranarray(P)
+        orc.lib().or_riemann_point(prm, U, U, Pf, Pf, np.asfortranarray(n), np.asfortranarray(t1),
+                                   np.asfortranarray(t2), F)
+        np.testing.assert_allclose(F, f @ n, rtol=1e-12, atol=1e-12)   # F*(u,u) = f(u).n
+        UR = random_state(rng)
+        PR = np.asfortranarray(euler_flux(UR)[1])
+        _, s1, s2 = orthonormal(n, rng)
+        G1, G2 = np.zeros(5, order="F"), np.zeros(5, order="F")
+        orc.lib().or_riemann_point(prm, U, UR, Pf, PR, np.asfortranarray(n), np.asfortranarray(t1),
+                                   np.asfortranarray(t2), G1)
+        orc.lib().or_riemann_point(prm, U, UR, Pf, PR, np.asfortranarray(n), np.asfortranarray(s1),
+                                   np.asfortranarray(s2), G2)
+        np.testing.assert_allclose(G1, G2, rtol=1e-12, atol=1e-12)      # any tangent basis
+
+
+def test_roe_matches_textbook_dissipation_on_1d_jump():
+    # Roe flux for a normal-velocity jump agrees with the explicit |A_roe| (U_R-U_L) of Toro ch. 11
+    rng = np.random.default_rng(4)
+    prm = orc.params(3, orc.NODE_GL, 0, orc.RIEMANN_ROE, **PRM)
+    g = 1.4
+    for _ in range(5):
+        UL, UR = random_state(rng), random_state(rng)
+        n = np.array([1.0, 0, 0]); t1 = np.array([0, 1.0, 0]); t2 = np.array([0, 0, 1.0])
+        fL, PL = euler_flux(UL)
+        fR, PR = euler_flux(UR)
+        F = np.zeros(5, order="F")
+        orc.lib().or_riemann_point(prm, UL, UR, np.asfortranarray(PL), np.asfortranarray(PR), n, t1, t2, F)
+        sL, sR = np.sqrt(UL[0]), np.sqrt(UR[0])
+        vL, vR = UL[1:4] / UL[0], UR[1:4] / UR[0]
+        HL, HR = (UL[4] + PL[4]) / UL[0], (UR[4] + PR[4]) / UR[0]
+        u = (sL * vL + sR * vR) / (sL + sR)
+        H = (sL * HL + sR * HR) / (sL + sR)
+        c = np.sqrt((g - 1) * (H - 0.5 * u @ u))
+        # Roe matrix A(u,H) in x with its eigen-decomposition (numerically)
+        uu = u @ u
+        A = np.zeros((5, 5))
+        A[0] = [0, 1, 0, 0, 0]
+        A[1] = [(g - 1) / 2 * uu - u[0] ** 2, (3 - g) * u[0], -(g - 1) * u[1], -(g - 1) * u[2], g - 1]
+        A[2] = [-u[0] * u[1], u[1], u[0], 0, 0]
+        A[3] = [-u[0] * u[2], u[2], 0, u[0], 0]
+        A[4] = [u[0] * ((g - 1) / 2 * uu - H), H - (g - 1) * u[0] ** 2, -(g - 1) * u[0] * u[1],
+                -(g - 1) * u[0] * u[2], g * u[0]]
+        w, V = np.linalg.eig(A)
+        absA = (V @ np.diag(np.abs(w)) @ np.linalg.inv(V)).real
+        ref = 0.5 * (fL[:, 0] + fR[:, 0]) - 0.5 * absA @ (UR - UL)
+        np.testing.assert_allclose(F, ref, rtol=1e-10, atol=1e-10)
+        assert np.isclose(np.sort(w.real)[0], u[0] - c) and np.isclose(np.sort(w.real)[-1], u[0] + c)
+
+
+@pytest.mark.parametrize("N,node,split", [(3, orc.NODE_G, 0), (5, orc.NODE_GL, 1), (4, orc.NODE_GL, 0)])
+def test_freestream_preserved_on_curved_mesh(N, node, split):
+    mesh = periodic_box(3, 2, 2, NGeo=N, deform=sine_deform(0.1))
+    prm = orc.params(N, node, split, orc.RIEMANN_ROE, **PRM)
+    m = orc.calc_metrics(N, node, mesh)
+    U = testcase.freestream(m["sJ"].shape)
+    Ut = orc.Solver(prm, mesh, m).rhs(U)
+    assert np.max(np.abs(Ut)) < 1e-11  # round-off: metric terms are sums of O(1e2) products
+
+
+@pytest.mark.parametrize("N,node,split", [(3, orc.NODE_G, 0), (5, orc.NODE_GL, 1)])
+def test_conservation_on_periodic_mesh(N, node, split):
+    # sum_k w_ijk J Ut = 0 for every conserved variable (telescoping surface fluxes)
+    mesh = periodic_box(3, 3, 2, NGeo=2, deform=sine_deform(0.1))
+    prm = orc.params(N, node, split, orc.RIEMANN_ROE, **PRM)
+    m = orc.calc_metrics(N, node, mesh)
+    U = testcase.tgv(m["Elem_xGP"], Ma=1.0)
+    Ut = orc.Solver(prm, mesh, m).rhs(U)
+    w = orc.nodes_weights(N, node)[1]
+    W = (w[:, None, None] * w[None, :, None] * w[None, None, :])[..., None] / m["sJ"]
+    for v in range(5):
+        tot = np.sum(W * Ut[v])
+        scale = np.sum(W * np.abs(Ut[v]))
+        assert abs(tot) <= 1e-13 * scale
+
+
+def test_calc_timestep_closed_form_uniform_flow():
+    N, nel = 4, 3
+    mesh = periodic_box(nel, nel, nel)
+    prm = orc.params(N, orc.NODE_GL, 1, orc.RIEMANN_ROE, **PRM)
+    m = orc.calc_metrics(N, orc.NODE_GL, mesh)
+    vel = np.array([0.3, 0.2, 0.1])
+    U = testcase.freestream(m["sJ"].shape, vel=tuple(vel), p=1.0)
+    cfl, dfl = orc.fill_cfl_dfl(N, orc.NODE_GL, 0.9, 0.9)
+    dt, err, _ = orc.Solver(prm, mesh, m).calc_timestep(U, cfl, dfl)
+    h2 = np.pi / nel  # h/2 on [-pi,pi]^3
+    c = np.sqrt(1.4)
+    lam = np.sum(np.abs(vel) / h2 + c / h2)
+    lamv = PRM["mu0"] * max(4 / 3, 1.4 / 0.72) * 3 / h2 ** 2
+    assert err == 0
+    assert np.isclose(dt, min(cfl * 2 / lam, dfl * 4 / lamv), rtol=1e-13)
+
+
+@pytest.mark.slow
+def test_oracle_reproduces_flexi_tgv_trajectory():
+    """regressioncheck/checks/tgv/split: N=7 GL, split PI, RoeEntropyFix, BR1, 8^3 on [0,2pi]^3,
+    CFL=DFL=0.8, analysis every 10 steps. The reference compares at relative 1e-4; the oracle
+    reproduces the first 20 steps to 1e-9 (columns near zero compared absolutely)."""
+    header, data = fx.tgv_split_csv()
+    mesh = periodic_box(8, 8, 8, lo=(0, 0, 0))
+    prm = orc.params(7, orc.NODE_GL, 1, orc.RIEMANN_ROEENTROPYFIX, **PRM)
+    m = orc.calc_metrics(7, orc.NODE_GL, mesh)
+    S = orc.Solver(prm, mesh, m)
+    U = testcase.tgv(m["Elem_xGP"])
+    S.rhs(U)
+    cfl, dfl = orc.fill_cfl_dfl(7, orc.NODE_GL, 0.8, 0.8)
+
+    def check(row, out):
+        for c, (got, ref) in enumerate(zip(out, row[1:])):
+            if abs(ref) < 1e-6:
+                assert abs(got - ref) < 1e-12, header[c + 1]
+            else:
+                assert abs(got - ref) <= 1e-9 * abs(ref), header[c + 1]
+
+    check(data[0], S.tgv_analyze(U, S.gradients()))
+    t, R = 0.0, orc.F(U.shape)
+    for it in range(1, 21):
+        dt, err, _ = S.calc_timestep(U, cfl, dfl)
+        S.step(U, t, dt, R)
+        t += dt
+        if it % 10 == 0:
+            row = data[it // 10]
+            assert abs(t - row[0]) <= 1e-13 * row[0]
+            check(row, S.tgv_analyze(U, S.gradients()))
