@@ -23,9 +23,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
-# Algorithmic HBM bytes per DOF of each kernel (FP64, N=7, Gauss-Lobatto); the derivation from
-# the per-DOF-stage byte model (432 B) is tabulated in DESIGN.md ("Byte model").
-ALGO_BYTES_PER_DOF = {"prolong_u": 60.0, "lift": 372.0, "flux": 147.0, "volint": 366.0}
+# Algorithmic HBM bytes per DOF of each kernel launch (FP64, N=7, Gauss-Lobatto: 512 volume
+# nodes, 296 boundary nodes and 6*64 own face points per element). Face values carry 40 B
+# (5 doubles), viscous face fluxes 32 B, metrics 80 B (Ja^1..3 + sJ).
+#   faces      : boundary U read 296*40 + own face states written 384*40       ->  53.1 B/DOF
+#   visc_faces : U 512*40 + boundary metrics 296*80 + neighbour face states 384*40
+#                + own viscous fluxes written 384*32                              -> 140.25 B/DOF
+#   dg_fused   : U 40 + metrics 80 + neighbour states 30 + neighbour fluxes 24 + U written 40
+#                + next face states 30 + register R read 40 (not in stage 1) + written 40,
+#                averaged over the 5 stages                                       -> 316.0 B/DOF
+# The four-kernel path's figures follow the same counting.
+ALGO_BYTES_PER_DOF = {"prolong_u": 60.0, "lift": 372.0, "flux": 147.0, "volint": 366.0,
+                      "faces": 53.125, "visc_faces": 140.25, "dg_fused": 316.0}
+# Per DOF-stage byte model of the stage (U, Ut/R, metrics and gradient traffic of one stage)
 B_STAGE = 432.0
 
 
@@ -61,8 +71,9 @@ def run_gpu(nel, precision, steps, warmup, profile):
     s.sync()
     rhs_wall = (time.perf_counter() - r0) / 50
     dof = mesh.nElems * 8 ** 3
+    pipeline = s.kernel_pipeline()
     s.close()
-    return dict(wall=wall, kernels=kt, dof=dof, rhs_s=rhs_wall, t=t)
+    return dict(wall=wall, kernels=kt, dof=dof, rhs_s=rhs_wall, t=t, pipeline=pipeline)
 
 
 def s_tgv(s):
@@ -130,7 +141,8 @@ def main():
     stages = 5
     value = r64["dof"] * stages * args.steps / r64["wall"]
     ms_per_step = 1e3 * r64["wall"] / args.steps
-    kern = {k: {"avg_ms": (v[0] / v[1] if v[1] else None), "launches": v[1]} for k, v in r64["kernels"].items()}
+    kern = {k: {"avg_ms": (v[0] / v[1] if v[1] else None), "launches": v[1]} for k, v in r64["kernels"].items()
+            if v[1]}
     # dominant kernel and its roofline (algorithmic bytes / measured average launch time)
     rhs_k = {k: v for k, v in r64["kernels"].items() if k in ALGO_BYTES_PER_DOF and v[1]}
     dom = max(rhs_k, key=lambda k: rhs_k[k][0])
@@ -153,7 +165,8 @@ def main():
         "data": "synthetic (Taylor-Green vortex initial state, Ma 0.1)",
         "config": {"workload": f"TGV {args.elems}^3 hex, N=7 Gauss-Lobatto, split-form PI, Roe, BR1, "
                                f"CarpenterRK4-5 with CALCTIMESTEP each step",
-                   "dof": r64["dof"], "stages_per_step": stages, "parallelism": "single GPU"},
+                   "dof": r64["dof"], "stages_per_step": stages, "parallelism": "single GPU",
+                   "pipeline": r64["pipeline"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_dof": ALGO_BYTES_PER_DOF[dom], "avg_launch_ms": avg_s * 1e3},

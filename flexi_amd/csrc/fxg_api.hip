@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -149,6 +150,10 @@ struct fxg_ctx {
   int* errHost = nullptr;
   std::vector<void*> owned;
   EventTimer* timer = nullptr;  // non-null while kernel timing is enabled
+  // fused Gauss-Lobatto split-form path: face states double-buffered across LSERK stages
+  bool fused = false;
+  fxg::FusedWork fw{};
+  int cur = 0;
 };
 
 namespace {
@@ -245,6 +250,17 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
     return 1;
   c->w.G = G; c->w.Gf_m = Gm; c->w.Gf_s = Gs;
   HIPCHECK(hipMemsetAsync(G, 0, 12 * vol * gsz, c->stream));
+  // FXG_PIPELINE=classic keeps the four-kernel path for cases the fused one covers
+  const char* pipe = std::getenv("FXG_PIPELINE");
+  c->fused = fxg::fused_supported(c->N, p->nodeType, p->splitDG == FXG_SPLITDG_PI) &&
+             !(pipe && std::strcmp(pipe, "classic") == 0);
+  if (c->fused) {
+    char* FG;
+    if (alloc_owned(c, &c->fw.FU[0], 10 * face) || alloc_owned(c, &c->fw.FU[1], 10 * face) ||
+        alloc_owned(c, &FG, 8 * face * gsz))
+      return 1;
+    c->fw.FG = FG;
+  }
   HIPCHECK(hipHostMalloc((void**)&c->dtHost, 2 * sizeof(double), hipHostMallocDefault));
   HIPCHECK(hipHostMalloc((void**)&c->errHost, sizeof(int), hipHostMallocDefault));
   c->m = fxg::DevMesh{c->N, K, S, c->E2S, c->s2v2, c->v2s2, c->Mf, c->Mg, c->Mh, c->sJ,
@@ -270,7 +286,15 @@ void ctx_free(fxg_ctx* c) {
   delete c;
 }
 
-int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStream_t s) {
+// One right-hand side (st.mode 0) or LSERK stage. On the fused path `faces_fresh` says that
+// FU[cur] already holds the face states of U (written by the previous stage).
+int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStream_t s, bool faces_fresh = false) {
+  if (c->fused) {
+    HIPCHECK(fxg::launch_rhs_fused(c->m, c->fw, c->gas, c->p.riemann, c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st,
+                                   faces_fresh, c->cur, st.mode == 0 ? c->w.G : nullptr, s, c->timer));
+    if (st.mode != 0) c->cur ^= 1;
+    return 0;
+  }
   HIPCHECK(fxg::launch_rhs(c->m, c->w, c->gas, c->p.nodeType, c->p.splitDG == FXG_SPLITDG_PI, c->p.riemann,
                            c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st, s, c->timer));
   return 0;
@@ -282,7 +306,7 @@ int lserk(fxg_ctx* c, double dt, int nStages, const double* RKA, const double* R
   if (!RKb || (nStages > 1 && !RKA)) return fail("RK coefficient arrays are NULL");
   for (int st = 0; st < nStages; ++st) {
     fxg::RKStage stage{st == 0 ? 1 : 2, st == 0 ? 0.0 : -RKA[st - 1], RKb[st] * dt, U, R};
-    if (rhs(c, U, nullptr, stage, s)) return 1;
+    if (rhs(c, U, nullptr, stage, s, st > 0)) return 1;
   }
   return 0;
 }
@@ -443,6 +467,8 @@ int fxg_timestep_lserkw2_dev(fxg_ctx* c, double t, double dt, int nStages, const
   if (!c || !U || !R) return fail("NULL argument");
   return lserk(c, dt, nStages, RKA, RKb, U, R, stream ? (hipStream_t)stream : c->stream);
 }
+
+const char* fxg_pipeline(fxg_ctx* c) { return !c ? "" : (c->fused ? "fused" : "classic"); }
 
 int fxg_sync(fxg_ctx* c) {
   if (!c) return fail("ctx is NULL");

@@ -1,0 +1,465 @@
+// Fused two-kernel stage for Gauss-Lobatto nodes and the split form (the benchmark path).
+//
+// Per stage, instead of prolongation / lifting / side fluxes / volume integral:
+//   A  k_visc_faces: per element, the BR1 gradients at its boundary nodes and, for each of its
+//      six faces, the viscous normal flux f_v . (n SurfElem) seen from that element (4 values per
+//      face point, momentum and energy) into the side array FG.
+//   B  k_dg_fused:   per element, the BR1 gradients at all nodes (recomputed, never stored), the
+//      face fluxes of its six sides (Riemann solver with the neighbour's face state, plus
+//      0.5*(own - neighbour) viscous normal fluxes), the split-form volume integral, Ut=-Ut*sJ,
+//      the LSERK register update and the new state's face values for the next stage.
+// Face states FU are double-buffered: stage s reads FU[s%2] and writes FU[(s+1)%2], so an element
+// never reads a face value that a neighbour is rewriting. Nothing is accumulated atomically.
+//
+// On Gauss-Lobatto nodes the face value of an element is its boundary node, and the face metric
+// n*SurfElem equals NormalSign*Ja^{NormalDir} at that node (src/mesh/metrics.f90:751-787), so both
+// sides of a face work from their own volume metrics; the two agree to round-off on conforming
+// meshes. Algebraically each element gets exactly FLEXI's contributions: master +Flux, slave -Flux
+// (src/dg/surfint.t90:471-616) with Flux = [F*(U_m,U_s;n_m) + (f_m+f_s).n_m/2]*SurfElem
+// (src/dg/fillflux.f90:40-215).
+#include "fxg_kernels.h"
+
+namespace fxg {
+namespace {
+
+template <int N>
+struct D {
+  static constexpr int n = N + 1, nf = n * n, np = n * n * n, NT = ((np + 63) / 64) * 64;
+};
+
+__device__ __forceinline__ bool minus_side(int ls) { return ls == 5 || ls == 2 || ls == 1; }
+__device__ __forceinline__ int dir_of(int ls) { return (ls == 5 || ls == 3) ? 0 : (ls == 2 || ls == 4) ? 1 : 2; }
+
+template <int N>
+__device__ __forceinline__ int vidx(int ls, int a, int b) {
+  constexpr int n = N + 1;
+  const int l = minus_side(ls) ? 0 : N;
+  switch (dir_of(ls)) {
+    case 0: return l + n * (a + n * b);
+    case 1: return a + n * (l + n * b);
+    default: return a + n * (b + n * l);
+  }
+}
+
+// Local sides a node lies on (bit ls-1 set) and its face coordinates.
+template <int N>
+__device__ __forceinline__ int node_sides(int i, int j, int k) {
+  int m = 0;
+  if (k == 0) m |= 1 << 0;   // zeta-
+  if (j == 0) m |= 1 << 1;   // eta-
+  if (i == N) m |= 1 << 2;   // xi+
+  if (j == N) m |= 1 << 3;   // eta+
+  if (i == 0) m |= 1 << 4;   // xi-
+  if (k == N) m |= 1 << 5;   // zeta+
+  return m;
+}
+__device__ __forceinline__ void face_ab(int ls, int i, int j, int k, int& a, int& b) {
+  switch (dir_of(ls)) {
+    case 0: a = j; b = k; break;
+    case 1: a = i; b = k; break;
+    default: a = i; b = j; break;
+  }
+}
+__device__ __forceinline__ double normal_sign(int ls) { return (ls == 3 || ls == 4 || ls == 6) ? 1.0 : -1.0; }
+
+// Side point (p,q) and slot (0 master, 1 slave) of own face (ls, a, b).
+struct FacePt {
+  size_t own;   // ((side*2 + ownslot)*nf + pq)
+  size_t nbr;   // ((side*2 + nbrslot)*nf + pq)
+  bool master;
+};
+template <int N>
+__device__ __forceinline__ FacePt face_pt(const DevMesh& m, int e, int ls, int a, int b) {
+  constexpr int n = N + 1, nf = n * n;
+  const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
+  const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
+  const int* pq = m.v2s2 + 2 * ((a + n * b) + nf * (flip + 5 * (ls - 1)));
+  const size_t p = (size_t)(pq[0] + n * pq[1]);
+  const bool master = flip == 0;
+  FacePt f;
+  f.master = master;
+  f.own = ((size_t)side * 2 + (master ? 0 : 1)) * nf + p;
+  f.nbr = ((size_t)side * 2 + (master ? 1 : 0)) * nf + p;
+  return f;
+}
+
+// BR1 lifting at node t (strong form, non-conservative): D_T contractions of the lifting
+// variables held in LDS (sP: u,v,w,T as SoA), metric transform, surface corrections
+// 0.5*(U_nb - U_own).(n SurfElem)*L_hat at face nodes, times sJ (lifting_br1.t90:48-181).
+template <int N, class RF>
+__device__ __forceinline__ void lift_node(const DevMesh& m, int e, int t, const double* sP, const double* sB,
+                                          const double* FU, const Gas& gas, const RF* Ja, RF sj, RF* g) {
+  constexpr int n = N + 1, np = n * n * n;
+  const BasisLayout B{n};
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  const double* DT = sB + B.D_T();
+  RF gxi[4], geta[4], gzeta[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) gxi[v] = geta[v] = gzeta[v] = RF(0);
+#pragma unroll
+  for (int l = 0; l <= N; ++l) {
+    const RF dxi = (RF)DT[l + n * i], deta = (RF)DT[l + n * j], dzeta = (RF)DT[l + n * k];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      gxi[v] += dxi * (RF)sP[v * np + (l + n * (j + n * k))];
+      geta[v] += deta * (RF)sP[v * np + (i + n * (l + n * k))];
+      gzeta[v] += dzeta * (RF)sP[v * np + (i + n * (j + n * l))];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) g[4 * d + v] = Ja[d] * gxi[v] + Ja[3 + d] * geta[v] + Ja[6 + d] * gzeta[v];
+  const int sides = node_sides<N>(i, j, k);
+  if (sides) {
+    const RF lhm = (RF)sB[B.L_HatMinus()], lhp = (RF)sB[B.L_HatPlus() + N];
+    const RF own[4] = {(RF)sP[t], (RF)sP[np + t], (RF)sP[2 * np + t], (RF)sP[3 * np + t]};
+#pragma unroll 1
+    for (int ls = 1; ls <= 6; ++ls) {
+      if (!(sides & (1 << (ls - 1)))) continue;
+      int a, b;
+      face_ab(ls, i, j, k, a, b);
+      const FacePt fp = face_pt<N>(m, e, ls, a, b);
+      const Prim q = cons_to_prim(FU + 5 * fp.nbr, gas);
+      const int dir = dir_of(ls);
+      const RF s = (RF)normal_sign(ls) * RF(0.5) * (minus_side(ls) ? lhm : lhp);
+      const RF nse[3] = {Ja[3 * dir] * s, Ja[3 * dir + 1] * s, Ja[3 * dir + 2] * s};
+      const RF dq[4] = {(RF)q.u - own[0], (RF)q.v - own[1], (RF)q.w - own[2], (RF)q.T - own[3]};
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) g[4 * d + v] += dq[v] * nse[d];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 12; ++c) g[c] *= sj;
+}
+
+template <int N, class RF>
+__device__ __forceinline__ void load_metrics(const DevMesh& m, size_t gid, RF* Ja, RF& sj) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    Ja[c] = (RF)m.Mf[3 * gid + c];
+    Ja[3 + c] = (RF)m.Mg[3 * gid + c];
+    Ja[6 + c] = (RF)m.Mh[3 * gid + c];
+  }
+  sj = (RF)m.sJ[gid];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernel A: viscous normal fluxes of the element's faces.
+template <int N, class RF>
+__global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restrict__ U, DevMesh m,
+                                                       const double* __restrict__ FU, RF* __restrict__ FG, Gas gas) {
+  constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT;
+  __shared__ double sP[4 * np];
+  __shared__ double sB[3 * n * n + 4 * n];
+  const int e = blockIdx.x, t = threadIdx.x;
+  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
+  if (t < np) {
+    const Prim q = cons_to_prim(U + 5 * ((size_t)np * e + t), gas);
+    sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
+  }
+  __syncthreads();
+  if (t >= np) return;
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  const int sides = node_sides<N>(i, j, k);
+  if (!sides) return;
+  const GasT<RF> gt(gas);
+  RF Ja[9], sj;
+  load_metrics<N, RF>(m, (size_t)np * e + t, Ja, sj);
+  RF g[12];
+  lift_node<N, RF>(m, e, t, sP, sB, FU, gas, Ja, sj, g);
+  RF fx[4], fy[4], fz[4];
+  diff_flux<RF>(gt, (RF)sP[t], (RF)sP[np + t], (RF)sP[2 * np + t], g, g + 4, g + 8, fx, fy, fz);
+#pragma unroll 1
+  for (int ls = 1; ls <= 6; ++ls) {
+    if (!(sides & (1 << (ls - 1)))) continue;
+    int a, b;
+    face_ab(ls, i, j, k, a, b);
+    const FacePt fp = face_pt<N>(m, e, ls, a, b);
+    const int dir = dir_of(ls);
+    const RF s = (RF)normal_sign(ls);
+    const RF n0 = Ja[3 * dir] * s, n1 = Ja[3 * dir + 1] * s, n2 = Ja[3 * dir + 2] * s;
+    RF* dst = FG + 4 * fp.own;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dst[c] = fx[c] * n0 + fy[c] * n1 + fz[c] * n2;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernel B: the rest of the stage.
+template <int N, class RF>
+__global__ __launch_bounds__(D<N>::NT) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
+                                                     double* __restrict__ FUnext, const RF* __restrict__ FG,
+                                                     Gas gas, int solver, double* Ut, RKStage st,
+                                                     RF* __restrict__ Gout) {
+  constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT;
+  // sA slots: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T (E after the lifting)
+  __shared__ double sA[7 * np];
+  // per-direction Ja^d (3) + viscous contravariant flux (4), or the surface scratch
+  __shared__ double sX[(7 * np > 54 * nf) ? 7 * np : 54 * nf];
+  __shared__ double sB[3 * n * n + 4 * n];
+  const BasisLayout B{n};
+  const GasT<RF> gt(gas);
+  const int e = blockIdx.x, t = threadIdx.x;
+  const bool act = t < np;
+  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
+  double Uown[5] = {0, 0, 0, 0, 0};
+  Prim q{};
+  const size_t gid = (size_t)np * e + (act ? t : 0);
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Uown[v] = U[5 * gid + v];
+    q = cons_to_prim(Uown, gas);
+    sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (Uown[4] + q.p) / Uown[0];
+    sA[3 * np + t] = q.u; sA[4 * np + t] = q.v; sA[5 * np + t] = q.w; sA[6 * np + t] = q.T;
+  }
+  __syncthreads();
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  RF Ja[9], sj;
+  RF fx[4] = {0, 0, 0, 0}, fy[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0};
+  if (act) {
+    load_metrics<N, RF>(m, gid, Ja, sj);
+    RF g[12];
+    lift_node<N, RF>(m, e, t, sA + 3 * np, sB, FU, gas, Ja, sj, g);  // u,v,w,T
+    diff_flux<RF>(gt, (RF)q.u, (RF)q.v, (RF)q.w, g, g + 4, g + 8, fx, fy, fz);
+    if (Gout) {  // (np, 12, nElems) like the four-kernel path
+#pragma unroll
+      for (int c = 0; c < 12; ++c) Gout[t + (size_t)np * (c + 12 * (size_t)e)] = g[c];
+    }
+  }
+  __syncthreads();  // T (sA slot 6) and all lifting reads are done
+  const int sides = act ? node_sides<N>(i, j, k) : 0;
+  double* sGE = sX;              // [6*nf][4]: own viscous normal flux per own face point
+  double* sSurf = sX + 24 * nf;  // [6*nf][5]: surface contribution per own face point
+  if (act) {
+    sA[6 * np + t] = Uown[4];  // E for the face states
+#pragma unroll 1
+    for (int ls = 1; ls <= 6; ++ls) {
+      if (!(sides & (1 << (ls - 1)))) continue;
+      int a, b;
+      face_ab(ls, i, j, k, a, b);
+      const int dir = dir_of(ls);
+      const RF s = (RF)normal_sign(ls);
+      const RF n0 = Ja[3 * dir] * s, n1 = Ja[3 * dir + 1] * s, n2 = Ja[3 * dir + 2] * s;
+      double* dst = sGE + 4 * ((ls - 1) * nf + a + n * b);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dst[c] = (double)(fx[c] * n0 + fy[c] * n1 + fz[c] * n2);
+    }
+  }
+  __syncthreads();
+  // surface phase: one lane per own face point
+  for (int x = t; x < 6 * nf; x += NT) {
+    const int ls = x / nf + 1, ab = x - (ls - 1) * nf;
+    const int a = ab % n, b = ab / n;
+    const int id = vidx<N>(ls, a, b);
+    const FacePt fp = face_pt<N>(m, e, ls, a, b);
+    const int dir = dir_of(ls);
+    const size_t nid = (size_t)np * e + id;
+    const double sgn = normal_sign(ls);
+    // own outward normal times SurfElem from the own metrics at the boundary node
+    const double* Jd = (dir == 0 ? m.Mf : (dir == 1 ? m.Mg : m.Mh)) + 3 * nid;
+    const double ns0 = sgn * Jd[0], ns1 = sgn * Jd[1], ns2 = sgn * Jd[2];
+    const double se = sqrt(ns0 * ns0 + ns1 * ns1 + ns2 * ns2);
+    const double ms = fp.master ? 1.0 : -1.0;
+    RF nv[3] = {(RF)(ms * ns0 / se), (RF)(ms * ns1 / se), (RF)(ms * ns2 / se)};  // master normal
+    // tangential basis: Gram-Schmidt of the coordinate axis least aligned with n
+    RF t1[3], t2[3];
+    {
+      const RF ax = fabs(nv[0]), ay = fabs(nv[1]), az = fabs(nv[2]);
+      RF h[3] = {RF(0), RF(0), RF(0)};
+      if (ax <= ay && ax <= az) h[0] = RF(1); else if (ay <= az) h[1] = RF(1); else h[2] = RF(1);
+      const RF dt_ = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
+      RF tt[3] = {h[0] - dt_ * nv[0], h[1] - dt_ * nv[1], h[2] - dt_ * nv[2]};
+      const RF tn = sqrt(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
+      t1[0] = tt[0] / tn; t1[1] = tt[1] / tn; t1[2] = tt[2] / tn;
+      t2[0] = nv[1] * t1[2] - nv[2] * t1[1];
+      t2[1] = nv[2] * t1[0] - nv[0] * t1[2];
+      t2[2] = nv[0] * t1[1] - nv[1] * t1[0];
+    }
+    const Prim qn = cons_to_prim(FU + 5 * fp.nbr, gas);
+    const double Enb = FU[5 * fp.nbr + 4];
+    const RotState<RF> So = rotate<RF>((RF)sA[id], (RF)sA[6 * np + id], (RF)sA[np + id], (RF)sA[3 * np + id],
+                                       (RF)sA[4 * np + id], (RF)sA[5 * np + id], nv, t1, t2);
+    const RotState<RF> Sn = rotate<RF>((RF)qn.rho, (RF)Enb, (RF)qn.p, (RF)qn.u, (RF)qn.v, (RF)qn.w, nv, t1, t2);
+    RF F[5];
+    if (fp.master) riemann_rot<RF>(gt, solver, true, So, Sn, F);
+    else riemann_rot<RF>(gt, solver, true, Sn, So, F);
+    RF Fo[5];
+    Fo[0] = F[0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Fo[1 + c] = nv[c] * F[1] + t1[c] * F[2] + t2[c] * F[3];
+    Fo[4] = F[4];
+    const double* ge = sGE + 4 * x;
+    const RF* gn = FG + 4 * fp.nbr;
+    const double lh = minus_side(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N];
+    double* dst = sSurf + 5 * x;
+    dst[0] = ms * (double)Fo[0] * se * lh;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      dst[1 + c] = (ms * (double)Fo[1 + c] * se + 0.5 * (ge[c] - (double)gn[c])) * lh;
+  }
+  __syncthreads();
+  RF acc[5] = {0, 0, 0, 0, 0};
+  if (sides) {
+#pragma unroll 1
+    for (int ls = 1; ls <= 6; ++ls) {
+      if (!(sides & (1 << (ls - 1)))) continue;
+      int a, b;
+      face_ab(ls, i, j, k, a, b);
+      const double* src = sSurf + 5 * ((ls - 1) * nf + a + n * b);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += (RF)src[v];
+    }
+  }
+  __syncthreads();  // sX is reused by the direction loop
+  // split-form volume integral with the viscous weak-form part (volint.f90:211-371)
+  const double* DV = sB + B.DVolSurf();
+  const double* DH = sB + B.D_Hat_T();
+  const RF rho = (RF)q.rho, uu = (RF)q.u, vv = (RF)q.v, ww = (RF)q.w, pp = (RF)q.p;
+  const RF HH = act ? (RF)sA[2 * np + t] : RF(0);
+  double* sM = sX;
+  double* sF = sX + 3 * np;
+#pragma unroll 1
+  for (int d = 0; d < 3; ++d) {
+    if (act) {
+      const RF* Jd = Ja + 3 * d;
+      sM[t] = (double)Jd[0]; sM[np + t] = (double)Jd[1]; sM[2 * np + t] = (double)Jd[2];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sF[c * np + t] = (double)(Jd[0] * fx[c] + Jd[1] * fy[c] + Jd[2] * fz[c]);
+    }
+    __syncthreads();
+    if (act) {
+      const int me = d == 0 ? i : (d == 1 ? j : k);
+      const int stride = d == 0 ? 1 : (d == 1 ? n : n * n);
+      const int base = t - me * stride;
+      const RF m0 = Ja[3 * d], m1 = Ja[3 * d + 1], m2 = Ja[3 * d + 2];
+#pragma unroll
+      for (int l = 0; l <= N; ++l) {
+        const int id = base + l * stride;
+        const RF dh = (RF)DH[l + n * me];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[1 + c] += dh * (RF)sF[c * np + id];
+        if (l == me) continue;
+        const RF rs = rho + (RF)sA[id];
+        const RF us = uu + (RF)sA[3 * np + id], vs = vv + (RF)sA[4 * np + id], ws = ww + (RF)sA[5 * np + id];
+        const RF ps = pp + (RF)sA[np + id], Hs = HH + (RF)sA[2 * np + id];
+        const RF a0 = RF(0.5) * (m0 + (RF)sM[id]), a1 = RF(0.5) * (m1 + (RF)sM[np + id]),
+                 a2 = RF(0.5) * (m2 + (RF)sM[2 * np + id]);
+        const RF mv = a0 * us + a1 * vs + a2 * ws;
+        const RF qq = RF(0.25) * rs * mv;
+        const RF dv = (RF)DV[l + n * me];
+        acc[0] += dv * (RF(0.5) * rs * mv);
+        acc[1] += dv * (qq * us + a0 * ps);
+        acc[2] += dv * (qq * vs + a1 * ps);
+        acc[3] += dv * (qq * ws + a2 * ps);
+        acc[4] += dv * (qq * Hs);
+      }
+    }
+    __syncthreads();
+  }
+  if (!act) return;
+  const double sjd = m.sJ[gid];
+  double ut[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) ut[v] = (-(double)acc[v]) * sjd;
+  const size_t o = 5 * gid;
+  if (st.mode == 0) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Ut[o + v] = ut[v];
+    return;
+  }
+  double un[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double r = (st.mode == 1) ? ut[v] : st.R[o + v] * st.minusA + ut[v];
+    st.R[o + v] = r;
+    un[v] = Uown[v] + r * st.b_dt;
+    st.U[o + v] = un[v];
+  }
+  // face values of the new state for the next stage
+#pragma unroll 1
+  for (int ls = 1; ls <= 6; ++ls) {
+    if (!(sides & (1 << (ls - 1)))) continue;
+    int a, b;
+    face_ab(ls, i, j, k, a, b);
+    const FacePt fp = face_pt<N>(m, e, ls, a, b);
+    double* dst = FUnext + 5 * fp.own;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) dst[v] = un[v];
+  }
+}
+
+// Face states of U into FU (all six faces of every element).
+template <int N>
+__global__ __launch_bounds__(D<N>::NT) void k_faces_of(const double* __restrict__ U, DevMesh m, double* __restrict__ FU) {
+  constexpr int n = D<N>::n, np = D<N>::np;
+  const int e = blockIdx.x, t = threadIdx.x;
+  if (t >= np) return;
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  const int sides = node_sides<N>(i, j, k);
+  if (!sides) return;
+  const double* src = U + 5 * ((size_t)np * e + t);
+  double u[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) u[v] = src[v];
+#pragma unroll 1
+  for (int ls = 1; ls <= 6; ++ls) {
+    if (!(sides & (1 << (ls - 1)))) continue;
+    int a, b;
+    face_ab(ls, i, j, k, a, b);
+    const FacePt fp = face_pt<N>(m, e, ls, a, b);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) FU[5 * fp.own + v] = u[v];
+  }
+}
+
+struct Scope {
+  KernelTimer* t;
+  int id;
+  hipStream_t s;
+  Scope(KernelTimer* t_, int id_, hipStream_t s_) : t(t_), id(id_), s(s_) { if (t) t->mark(id, true, s); }
+  ~Scope() { if (t) t->mark(id, false, s); }
+};
+
+template <int N, class RF>
+hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, const double* U, double* Ut,
+                   const RKStage& st, bool faces_fresh, int cur, void* G, hipStream_t s, KernelTimer* tm) {
+  const int K = m.nElems;
+  constexpr int NT = D<N>::NT;
+  if (!faces_fresh) {
+    Scope sc(tm, K_FACES, s);
+    hipLaunchKernelGGL((k_faces_of<N>), dim3(K), dim3(NT), 0, s, U, m, w.FU[cur]);
+  }
+  {
+    Scope sc(tm, K_VISC_FACES, s);
+    hipLaunchKernelGGL((k_visc_faces<N, RF>), dim3(K), dim3(NT), 0, s, U, m, w.FU[cur], (RF*)w.FG, gas);
+  }
+  {
+    Scope sc(tm, K_DG_FUSED, s);
+    hipLaunchKernelGGL((k_dg_fused<N, RF>), dim3(K), dim3(NT), 0, s, U, m, w.FU[cur], w.FU[1 - cur], (const RF*)w.FG,
+                       gas, riemann, Ut, st, (RF*)G);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, int fp32,
+                            const double* U, double* Ut, const RKStage& st, bool faces_fresh, int cur, void* G,
+                            hipStream_t s, KernelTimer* tm) {
+  switch (m.N) {
+#define FXG_CASE(NN)                                                                                   \
+  case NN:                                                                                             \
+    return fp32 ? fused_n<NN, float>(m, w, gas, riemann, U, Ut, st, faces_fresh, cur, G, s, tm)           \
+                : fused_n<NN, double>(m, w, gas, riemann, U, Ut, st, faces_fresh, cur, G, s, tm);
+    FXG_CASE(2) FXG_CASE(3) FXG_CASE(4) FXG_CASE(5) FXG_CASE(6) FXG_CASE(7)
+#undef FXG_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool fused_supported(int N, int nodeType, int split) { return nodeType == 2 && split && N >= 2 && N <= 7; }
+
+}  // namespace fxg

@@ -63,7 +63,11 @@ struct RKStage {
 };
 
 // Kernel ids for the optional event timer (fxg_profile_read).
-enum KernelId { K_PROLONG = 0, K_LIFT = 1, K_FLUX = 2, K_VOLINT = 3, K_DT_ELEM = 4, K_DT_REDUCE = 5, K_COUNT = 6 };
+// 0-3: four-kernel path; 6-8: fused Gauss-Lobatto split-form path.
+enum KernelId {
+  K_PROLONG = 0, K_LIFT = 1, K_FLUX = 2, K_VOLINT = 3, K_DT_ELEM = 4, K_DT_REDUCE = 5,
+  K_FACES = 6, K_VISC_FACES = 7, K_DG_FUSED = 8, K_COUNT = 9
+};
 
 // Records a HIP event before (start = true) and after each kernel launch when timing is on.
 struct KernelTimer {
@@ -76,6 +80,20 @@ struct KernelTimer {
 hipError_t launch_rhs(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, int split,
                       int riemann, int fp32, const double* U, double* Ut, const RKStage& stage,
                       hipStream_t s, KernelTimer* timer);
+// Work arrays of the fused Gauss-Lobatto split-form stage (fxg_fused.hip). Face arrays are
+// indexed ((side*2 + slot)*(N+1)^2 + pq) with slot 0 = master, 1 = slave.
+struct FusedWork {
+  double* FU[2];  // face states, 5 per side point and slot, double-buffered across stages
+  void* FG;       // viscous normal face fluxes f_v.(n SurfElem), 4 per side point and slot
+};
+bool fused_supported(int N, int nodeType, int split);
+// One stage of the fused path. FU[cur] must hold the face states of U unless faces_fresh is
+// false, in which case they are rebuilt first. In RK modes the new state's faces go to FU[1-cur].
+// G (optional, (np,12,nElems) of the gradient type) receives the lifted gradients.
+hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, int fp32,
+                            const double* U, double* Ut, const RKStage& st, bool faces_fresh, int cur,
+                            void* G, hipStream_t s, KernelTimer* timer);
+
 hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U,
                                 double CFL, double DFL, hipStream_t s, KernelTimer* timer);
 hipError_t launch_metrics_adv(const DevMesh& m, double KappasPr_max, double* adv, double* visc,

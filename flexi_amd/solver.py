@@ -167,6 +167,10 @@ class DGSolver:
         _lib.check(_lib.lib().fxg_timestep_lserkw2_dev(self.ctx, float(t), float(dt), self.scheme.nStages, self._RKA,
                                                        self._RKb, self._RKc, U_ptr, R_ptr, stream or None))
 
+    def kernel_pipeline(self) -> str:
+        """'fused' or 'classic' (which stage kernels this context launches)."""
+        return _lib.lib().fxg_pipeline(self.ctx).decode()
+
     def sync(self) -> None:
         _lib.check(_lib.lib().fxg_sync(self.ctx))
 

@@ -100,8 +100,11 @@ int fxg_destroy(fxg_ctx* ctx);
 int fxg_set_state(fxg_ctx* ctx, const double* U);
 int fxg_get_state(fxg_ctx* ctx, double* U);
 int fxg_get_ut(fxg_ctx* ctx, double* Ut);
-/* Lifted gradients of the last right-hand side, gradU{x,y,z}(4,0:N,0:N,0:N,nElems) for the
- * lifting variables (u,v,w,T) (src/dg/lifting/lifting_vars.f90; PP_OPTLIFT=1 set, eos.h:114). */
+/* Lifted gradients of the last fxg_dg_time_derivative(_dev) call,
+ * gradU{x,y,z}(4,0:N,0:N,0:N,nElems) for the lifting variables (u,v,w,T)
+ * (src/dg/lifting/lifting_vars.f90; PP_OPTLIFT=1 set, eos.h:114). LSERK stages do not store
+ * them; FLEXI's analysis evaluates a fresh right-hand side before it reads them
+ * (src/timedisc/timedisc_func.f90:324-325). */
 int fxg_get_gradients(fxg_ctx* ctx, double* gx, double* gy, double* gz);
 /* Geometry as used on the device (for checking fxg_create_from_nodes): same layouts as fxg_mesh
  * plus Elem_xGP(3,0:N,0:N,0:N,nElems). Any pointer may be NULL. */
@@ -130,6 +133,11 @@ int fxg_dg_time_derivative_dev(fxg_ctx* ctx, double t, const double* U_dev, doub
 int fxg_timestep_lserkw2_dev(fxg_ctx* ctx, double t, double dt, int nStages, const double* RKA,
                              const double* RKb, const double* RKc, double* U_dev, double* R_dev,
                              void* stream);
+/* Which stage kernels the context runs: "fused" (Gauss-Lobatto split form, N 2..7: two
+ * kernels per stage, face states carried between LSERK stages) or "classic" (prolongation,
+ * lifting, side fluxes, integrals). No counterpart in the reference; FXG_PIPELINE=classic in
+ * the environment at fxg_create time selects the classic path everywhere. */
+const char* fxg_pipeline(fxg_ctx* ctx);
 /* Synchronise the context's stream. */
 int fxg_sync(fxg_ctx* ctx);
 
@@ -141,7 +149,9 @@ int fxg_sizes(fxg_ctx* ctx, int* N, int* nElems, int* nSides);
  * event pair brackets every kernel launch on the launch stream; fxg_profile_read synchronises
  * and returns the summed duration and launch count of one kernel since enabling.
  * Kernel ids: 0 prolongation, 1 lifting, 2 side fluxes, 3 volume/surface integral (+RK update),
- * 4 per-element time step, 5 time-step reduction. Enabling again resets the counters. */
+ * 4 per-element time step, 5 time-step reduction; on the fused Gauss-Lobatto split-form path
+ * 6 face states of the first stage, 7 viscous face fluxes, 8 fused stage. Enabling again resets
+ * the counters. */
 int fxg_profile_enable(fxg_ctx* ctx, int enable);
 int fxg_profile_read(fxg_ctx* ctx, int kernel, double* total_ms, int* launches);
 

@@ -137,3 +137,24 @@ def test_density_perturbation_is_seeded_and_bounded():
     np.testing.assert_array_equal(U1, U2)
     assert not np.array_equal(U1, U3)
     assert np.all(np.abs(U1[0] - 1.0) <= 1e-3) and np.std(U1[0]) > 1e-4
+
+
+def test_fortran_binding_compiles_links_and_reports_errors(tmp_path):
+    """flexi_amd/fortran/fxg_interface.f90 (the module a FLEXI host USEs) against libfxg.so."""
+    import shutil
+    from flexi_amd import _lib
+    fc = shutil.which("amdflang") or shutil.which("flang")
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    _lib.lib()
+    mod = os.path.join(ROOT, "flexi_amd", "fortran", "fxg_interface.f90")
+    drv = os.path.join(ROOT, "tests", "fortran", "fxg_abi_check.f90")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run([fc, "-c", mod, "-o", str(tmp_path / "m.o"), f"-J{tmp_path}"], check=True, cwd=tmp_path)
+    subprocess.run([fc, drv, str(tmp_path / "m.o"), f"-I{tmp_path}", f"-L{libdir}", "-lfxg",
+                    f"-Wl,-rpath,{libdir}", "-o", str(tmp_path / "chk")], check=True, cwd=tmp_path)
+    out = subprocess.run([str(tmp_path / "chk")], capture_output=True, text=True, check=True).stdout
+    assert out.strip().endswith("OK"), out
+    sizes = dict(line.split("=") for line in out.splitlines() if line.startswith("sizeof"))
+    assert int(sizes["sizeof(fxg_params)"]) == ctypes.sizeof(_lib.FxgParams)
+    assert int(sizes["sizeof(fxg_mesh)"]) == ctypes.sizeof(_lib.FxgMesh)

@@ -125,3 +125,68 @@ def test_nan_density_reports_errtype():
     gpu.set_state(U)
     dt, err = gpu.CalcTimeStep()
     assert err == 1
+
+
+# The Gauss-Lobatto split-form configurations run the fused two-kernel stage by default;
+# FXG_PIPELINE=classic selects the four-kernel path for the same context.
+SPLIT_CASES = [
+    (2, orc.RIEMANN_ROE),
+    (3, orc.RIEMANN_LF),
+    (5, orc.RIEMANN_ROEENTROPYFIX),
+    (7, orc.RIEMANN_ROE),
+]
+
+
+@pytest.mark.parametrize("N,riemann", SPLIT_CASES)
+def test_classic_pipeline_matches_oracle(N, riemann, monkeypatch):
+    monkeypatch.setenv("FXG_PIPELINE", "classic")
+    mesh, m, oracle, gpu, U = setup(N, orc.NODE_GL, True, riemann)
+    assert rel(gpu.rhs(U), oracle.rhs(U)) <= TOL
+    gpu.close()
+
+
+@pytest.mark.parametrize("N,riemann", SPLIT_CASES)
+def test_fused_pipeline_curvilinear_and_gradients(N, riemann):
+    mesh, m, oracle, gpu, U = setup(N, orc.NODE_GL, True, riemann, shape=(3, 2, 2), deform=sine_deform(0.1), NGeo=N)
+    assert gpu.kernel_pipeline() == "fused"
+    assert rel(gpu.rhs(U), oracle.rhs(U)) <= TOL
+    for a, b in zip(gpu.gradients(), oracle.gradients()):
+        assert rel(a, b) <= TOL
+
+
+def test_fused_lserk_reuses_faces_across_stages():
+    # the faces written by stage s feed stage s+1; a device-pointer step with caller buffers and a
+    # host-state step must agree with the oracle and with each other
+    import torch
+    mesh, m, oracle, gpu, U = setup(7, orc.NODE_GL, True, orc.RIEMANN_ROE, shape=(2, 3, 2))
+    dt = 2e-3
+    Uo = np.asfortranarray(U.copy())
+    R = orc.F(U.shape)
+    for it in range(4):
+        oracle.step(Uo, it * dt, dt, R)
+    gpu.set_state(U)
+    gpu.dt = dt
+    for it in range(4):
+        gpu.TimeStepByLSERKW2(it * dt)
+    Ug = gpu.get_state()
+    assert rel(Ug - U, Uo - U) <= 1e-11
+    dev = torch.device("cuda:0")
+    Ud = torch.from_numpy(U.ravel(order="F").copy()).to(dev)
+    Rd = torch.zeros_like(Ud)
+    torch.cuda.synchronize()  # the context's stream does not wait for torch's null stream
+    for it in range(4):
+        gpu.step_dev(Ud.data_ptr(), Rd.data_ptr(), dt, it * dt)
+    gpu.sync()
+    Uh = Ud.cpu().numpy().reshape(U.shape, order="F")
+    assert rel(Uh - U, Uo - U) <= 1e-11
+
+
+def test_fused_rhs_conserves_on_periodic_mesh():
+    mesh, m, oracle, gpu, U = setup(5, orc.NODE_GL, True, orc.RIEMANN_ROEENTROPYFIX, shape=(3, 3, 2),
+                                    deform=sine_deform(0.1), NGeo=2)
+    Ut = gpu.rhs(U)
+    w = orc.nodes_weights(5, orc.NODE_GL)[1]
+    W = (w[:, None, None] * w[None, :, None] * w[None, None, :])[..., None] / m["sJ"]
+    for v in range(5):
+        # the two sides of a face take n*SurfElem from their own metrics, equal to round-off
+        assert abs(np.sum(W * Ut[v])) <= 1e-13 * np.sum(W * np.abs(Ut[v]))
