@@ -38,6 +38,13 @@ def lib() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FxgError(f"{LIB_PATH} is missing: build it with `python -m flexi_amd.build` (no CPU fallback exists)")
+    # torch's ROCm wheel carries its own libamdhip64.so.7. Loading it first lets libfxg.so bind to
+    # that copy (same SONAME), so one HIP runtime serves both and device pointers can be shared;
+    # with /opt/rocm's copy loaded first, torch would start a second runtime that sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
     L.fxg_last_error.restype = C.c_char_p

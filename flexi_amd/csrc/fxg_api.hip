@@ -264,7 +264,7 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   HIPCHECK(hipHostMalloc((void**)&c->dtHost, 2 * sizeof(double), hipHostMallocDefault));
   HIPCHECK(hipHostMalloc((void**)&c->errHost, sizeof(int), hipHostMallocDefault));
   c->m = fxg::DevMesh{c->N, K, S, c->E2S, c->s2v2, c->v2s2, c->Mf, c->Mg, c->Mh, c->sJ,
-                      c->NV, c->T1, c->T2, c->SE, c->adv, c->visc, c->basis};
+                      c->NV, c->T1, c->T2, c->SE, c->adv, c->visc, c->basis, 0};
   return 0;
 }
 
@@ -363,6 +363,9 @@ int fxg_create_from_nodes(const fxg_params* p, int nElems, int nSides, const int
                                c->T1, c->T2, c->SE, c->stream, err)) {
     fail(err); ctx_free(c); return 1;
   }
+  // the curl-form face metric depends only on the face, so each element's own boundary metric
+  // is the face metric; using it on both sides keeps the freestream exact to flux round-off
+  c->m.ownFaceMetric = 1;
   if (finish_setup(c)) { ctx_free(c); return 1; }
   *out = c;
   return 0;

@@ -12,11 +12,15 @@
 // never reads a face value that a neighbour is rewriting. Nothing is accumulated atomically.
 //
 // On Gauss-Lobatto nodes the face value of an element is its boundary node, and the face metric
-// n*SurfElem equals NormalSign*Ja^{NormalDir} at that node (src/mesh/metrics.f90:751-787), so both
-// sides of a face work from their own volume metrics; the two agree to round-off on conforming
-// meshes. Algebraically each element gets exactly FLEXI's contributions: master +Flux, slave -Flux
-// (src/dg/surfint.t90:471-616) with Flux = [F*(U_m,U_s;n_m) + (f_m+f_s).n_m/2]*SurfElem
-// (src/dg/fillflux.f90:40-215).
+// n*SurfElem equals NormalSign*Ja^{NormalDir} at that node (src/mesh/metrics.f90:751-787). With
+// host-supplied metrics the Riemann flux uses the side's NormVec/SurfElem on both sides, as FLEXI
+// does; with metrics computed by the library each element uses its own boundary metric (the
+// curl-form face metric depends only on the face, so the two agree to round-off, and the volume and
+// surface terms of an element then cancel exactly for a uniform state). The viscous normal fluxes
+// and the lifting corrections always take n*SurfElem from the element's own metric terms (equal to
+// round-off, and multiplied by small quantities there). Each element gets FLEXI's contributions:
+// master +Flux, slave -Flux (src/dg/surfint.t90:471-616) with
+// Flux = [F*(U_m,U_s;n_m) + (f_m+f_s).n_m/2]*SurfElem (src/dg/fillflux.f90:40-215).
 #include "fxg_kernels.h"
 
 namespace fxg {
@@ -66,6 +70,7 @@ __device__ __forceinline__ double normal_sign(int ls) { return (ls == 3 || ls ==
 struct FacePt {
   size_t own;   // ((side*2 + ownslot)*nf + pq)
   size_t nbr;   // ((side*2 + nbrslot)*nf + pq)
+  size_t sp;    // side*nf + pq, the side-array index (NormVec, SurfElem)
   bool master;
 };
 template <int N>
@@ -80,6 +85,7 @@ __device__ __forceinline__ FacePt face_pt(const DevMesh& m, int e, int ls, int a
   f.master = master;
   f.own = ((size_t)side * 2 + (master ? 0 : 1)) * nf + p;
   f.nbr = ((size_t)side * 2 + (master ? 1 : 0)) * nf + p;
+  f.sp = (size_t)side * nf + p;
   return f;
 }
 
@@ -255,15 +261,22 @@ __global__ __launch_bounds__(D<N>::NT) void k_dg_fused(const double* U, DevMesh 
     const int a = ab % n, b = ab / n;
     const int id = vidx<N>(ls, a, b);
     const FacePt fp = face_pt<N>(m, e, ls, a, b);
-    const int dir = dir_of(ls);
-    const size_t nid = (size_t)np * e + id;
-    const double sgn = normal_sign(ls);
-    // own outward normal times SurfElem from the own metrics at the boundary node
-    const double* Jd = (dir == 0 ? m.Mf : (dir == 1 ? m.Mg : m.Mh)) + 3 * nid;
-    const double ns0 = sgn * Jd[0], ns1 = sgn * Jd[1], ns2 = sgn * Jd[2];
-    const double se = sqrt(ns0 * ns0 + ns1 * ns1 + ns2 * ns2);
     const double ms = fp.master ? 1.0 : -1.0;
-    RF nv[3] = {(RF)(ms * ns0 / se), (RF)(ms * ns1 / se), (RF)(ms * ns2 / se)};  // master normal
+    double se;
+    RF nv[3];  // master normal
+    if (m.ownFaceMetric) {
+      // own outward normal times SurfElem from the own metric terms at the boundary node
+      const int dir = dir_of(ls);
+      const double sgn = normal_sign(ls);
+      const double* Jd = (dir == 0 ? m.Mf : (dir == 1 ? m.Mg : m.Mh)) + 3 * ((size_t)np * e + id);
+      const double ns0 = sgn * Jd[0], ns1 = sgn * Jd[1], ns2 = sgn * Jd[2];
+      se = sqrt(ns0 * ns0 + ns1 * ns1 + ns2 * ns2);
+      nv[0] = (RF)(ms * ns0 / se); nv[1] = (RF)(ms * ns1 / se); nv[2] = (RF)(ms * ns2 / se);
+    } else {
+      // the side's normal and surface element, shared by master and slave as in FLEXI
+      se = m.SurfElem[fp.sp];
+      nv[0] = (RF)m.NormVec[3 * fp.sp]; nv[1] = (RF)m.NormVec[3 * fp.sp + 1]; nv[2] = (RF)m.NormVec[3 * fp.sp + 2];
+    }
     // tangential basis: Gram-Schmidt of the coordinate axis least aligned with n
     RF t1[3], t2[3];
     {

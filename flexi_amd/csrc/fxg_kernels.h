@@ -37,6 +37,10 @@ struct DevMesh {
   const double* MetricsAdv;   // (3,0:N,0:N,0:N,nElems), calctimestep.f90:42-83
   const double* MetricsVisc;  // (3,0:N,0:N,0:N,nElems)
   const double* basis;        // BasisLayout
+  // Gauss-Lobatto fused stage: 1 = each element takes n*SurfElem of a face from its own metric
+  // terms at the face node (metrics computed by the library), 0 = the side's NormVec/SurfElem as
+  // supplied by the host, shared by master and slave as FLEXI does
+  int ownFaceMetric;
 };
 
 // Work arrays of one right-hand side evaluation. Face arrays of U are FP64 (U(5,p,q,side));

@@ -175,8 +175,12 @@ __global__ __launch_bounds__(512) void k_metrics_n(const double* __restrict__ XC
           d[2 + 3 * c] += DN[k + n * l] * xe[3 * vol_id(n, i, j, l) + c];
         }
     }
-    // R_CL_N(:,1) = (z dY - y dZ)/2, R(:,2) = (x dZ - z dX)/2, R(:,3) = (y dX - x dY)/2  (metrics.f90:443-447)
-    const double X = xe[3 * o], Y = xe[3 * o + 1], Z = xe[3 * o + 2];
+    // R_CL_N(:,1) = (z dY - y dZ)/2, R(:,2) = (x dZ - z dX)/2, R(:,3) = (y dX - x dY)/2  (metrics.f90:443-447).
+    // Coordinates are taken relative to the element's first CL node: the discrete curl of
+    // c*grad(X) vanishes for constant c (the tensor-product derivatives commute), so the metric
+    // terms are unchanged, but the products stay O(h^2) instead of O(|x| h). Translated copies
+    // of an element (periodic neighbours) then get the same face metrics to round-off of h^2.
+    const double X = xe[3 * o] - xe[0], Y = xe[3 * o + 1] - xe[1], Z = xe[3 * o + 2] - xe[2];
     for (int dd = 0; dd < 3; ++dd) {
       sR[(dd + 3 * 0) * np + o] = 0.5 * (Z * d[dd + 3 * 1] - Y * d[dd + 3 * 2]);
       sR[(dd + 3 * 1) * np + o] = 0.5 * (X * d[dd + 3 * 2] - Z * d[dd + 3 * 0]);

@@ -1,0 +1,80 @@
+"""GPU checks of whole runs: FP32-bulk accuracy against the FP64 run (Taylor-Green vortex, 8^3,
+N=3 Gauss, weak form, LF, 1000 steps) and freestream preservation on the curvilinear 24^3 N=5
+mesh with metrics computed on the GPU."""
+import numpy as np
+import pytest
+
+from flexi_amd import testcase
+from flexi_amd.mesh import periodic_box, sine_deform
+
+pytestmark = pytest.mark.gpu
+
+
+def weights(N, node):
+    import oracle as orc
+    return orc.nodes_weights(N, node)[1]
+
+
+def ekin_rate(U, Ut, W):
+    """dEkin/dt = sum w J (u . (rho u)_t - |u|^2/2 rho_t) with Ekin = sum w J rho |u|^2 / 2."""
+    rho = U[0]
+    u = U[1:4] / rho
+    return np.sum(W * (np.sum(u * Ut[1:4], axis=0) - 0.5 * np.sum(u * u, axis=0) * Ut[0]))
+
+
+def test_fp32_bulk_tgv_1000_steps_against_fp64():
+    from flexi_amd.solver import DGSolver
+    import oracle as orc
+    N = 3
+    mesh = periodic_box(8, 8, 8)
+    runs = {}
+    dt = None
+    for prec in ("fp64", "fp32-bulk"):
+        s = DGSolver(mesh, N, node_type="gauss", split_dg=None, riemann="lf", precision=prec, CFLScale=0.9,
+                     DFLScale=0.9)
+        m = s.metrics()
+        U0 = testcase.tgv_density_perturbed(m["Elem_xGP"], amplitude=1e-3, seed=1)
+        s.set_state(U0)
+        if dt is None:
+            dt, err = s.CalcTimeStep()
+            assert err == 0
+        s.run(1000, fixed_dt=dt)  # the FP64 run's first dt for both, so both reach the same time
+        U = s.get_state()
+        s.DGTimeDerivative_weakForm(1000 * dt)
+        runs[prec] = (U, s.get_ut(), m["sJ"])
+        s.close()
+    (U64, Ut64, sJ), (U32, Ut32, _) = runs["fp64"], runs["fp32-bulk"]
+    w = weights(N, orc.NODE_G)
+    W = (w[:, None, None] * w[None, :, None] * w[None, None, :])[..., None] / sJ
+    err_U = np.linalg.norm((U32 - U64).ravel()) / np.linalg.norm(U64.ravel())
+    d64, d32 = -ekin_rate(U64, Ut64, W), -ekin_rate(U32, Ut32, W)
+    err_eps = abs(d32 - d64) / abs(d64)
+    print(f"FP32-bulk after 1000 steps (t={1000 * dt:.4f}): rel L2 U {err_U:.3e}, "
+          f"-dEkin/dt {d64:.6e} vs {d32:.6e} (rel {err_eps:.3e})")
+    assert np.all(np.isfinite(U32))
+    assert err_U <= 1e-6
+    assert err_eps <= 1e-3
+
+
+@pytest.mark.parametrize("pipeline", ["fused", "classic"])
+def test_curvilinear_freestream_24cubed_n5_gpu_metrics(pipeline, monkeypatch):
+    from flexi_amd.solver import DGSolver
+    if pipeline == "classic":
+        monkeypatch.setenv("FXG_PIPELINE", "classic")
+    N = 5
+    mesh = periodic_box(24, 24, 24, NGeo=N, deform=sine_deform(0.1))
+    s = DGSolver(mesh, N, node_type="gauss-lobatto", split_dg="pi", riemann="roe")
+    assert s.kernel_pipeline() == pipeline
+    n = N + 1
+    U0 = testcase.freestream((n, n, n, mesh.nElems), vel=(0.3, 0.2, 0.1), p=1.0 / 1.4)
+    s.set_state(U0)
+    s.run(100)
+    U = s.get_state()
+    err = np.max(np.abs(U - U0))
+    print(f"freestream 24^3 N=5 curvilinear ({pipeline}), 100 steps to t={s.t:.4f}: max|U-U0| = {err:.3e} "
+          f"(per variable {np.abs(U - U0).reshape(5, -1).max(1)})")
+    # Fused stage with library-computed metrics: each element's face metric is its own boundary
+    # metric, so volume and surface terms cancel to flux round-off. The four-kernel stage follows
+    # FLEXI and gives the slave the master's NormVec*SurfElem, which differs from the slave's own
+    # boundary metric by the round-off of two separate metric evaluations (~2e-12 here).
+    assert err <= (1e-13 if pipeline == "fused" else 1e-11)
