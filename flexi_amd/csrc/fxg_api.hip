@@ -252,7 +252,7 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   HIPCHECK(hipMemsetAsync(G, 0, 12 * vol * gsz, c->stream));
   // FXG_PIPELINE=classic keeps the four-kernel path for cases the fused one covers
   const char* pipe = std::getenv("FXG_PIPELINE");
-  c->fused = fxg::fused_supported(c->N, p->nodeType, p->splitDG == FXG_SPLITDG_PI) &&
+  c->fused = fxg::fused_supported(c->N, p->nodeType, p->splitDG == FXG_SPLITDG_PI, S) &&
              !(pipe && std::strcmp(pipe, "classic") == 0);
   if (c->fused) {
     char* FG;

@@ -4,12 +4,16 @@
 //   A  k_visc_faces: per element, the BR1 gradients at its boundary nodes and, for each of its
 //      six faces, the viscous normal flux f_v . (n SurfElem) seen from that element (4 values per
 //      face point, momentum and energy) into the side array FG.
-//   B  k_dg_fused:   per element, the BR1 gradients at all nodes (recomputed, never stored), the
-//      face fluxes of its six sides (Riemann solver with the neighbour's face state, plus
-//      0.5*(own - neighbour) viscous normal fluxes), the split-form volume integral, Ut=-Ut*sJ,
-//      the LSERK register update and the new state's face values for the next stage.
+//   B  k_dg_fused:   per element, the face fluxes of its six sides (Riemann solver with the
+//      neighbour's face state, plus 0.5*(own - neighbour) viscous normal fluxes), the BR1
+//      gradients at all nodes (recomputed, never stored), the split-form volume integral,
+//      Ut=-Ut*sJ, the LSERK register update and the new state's face values for the next stage.
 // Face states FU are double-buffered: stage s reads FU[s%2] and writes FU[(s+1)%2], so an element
 // never reads a face value that a neighbour is rewriting. Nothing is accumulated atomically.
+//
+// Both kernels first stage, per own face point, the side-array index and the neighbour's face
+// state in LDS (one lane per face point); every later phase reads them from there, so the only
+// dependent global loads (ElemToSide -> S2V2 -> face array) happen once, at the start.
 //
 // On Gauss-Lobatto nodes the face value of an element is its boundary node, and the face metric
 // n*SurfElem equals NormalSign*Ja^{NormalDir} at that node (src/mesh/metrics.f90:751-787). With
@@ -29,11 +33,14 @@ namespace {
 template <int N>
 struct D {
   static constexpr int n = N + 1, nf = n * n, np = n * n * n, NT = ((np + 63) / 64) * 64;
+  static constexpr int NF = 6 * nf;  // own face points of an element
 };
 
 __device__ __forceinline__ bool minus_side(int ls) { return ls == 5 || ls == 2 || ls == 1; }
 __device__ __forceinline__ int dir_of(int ls) { return (ls == 5 || ls == 3) ? 0 : (ls == 2 || ls == 4) ? 1 : 2; }
+__device__ __forceinline__ double normal_sign(int ls) { return (ls == 3 || ls == 4 || ls == 6) ? 1.0 : -1.0; }
 
+// Volume node of own face point (ls, a, b).
 template <int N>
 __device__ __forceinline__ int vidx(int ls, int a, int b) {
   constexpr int n = N + 1;
@@ -45,7 +52,7 @@ __device__ __forceinline__ int vidx(int ls, int a, int b) {
   }
 }
 
-// Local sides a node lies on (bit ls-1 set) and its face coordinates.
+// Local sides a node lies on (bit ls-1 set).
 template <int N>
 __device__ __forceinline__ int node_sides(int i, int j, int k) {
   int m = 0;
@@ -57,45 +64,49 @@ __device__ __forceinline__ int node_sides(int i, int j, int k) {
   if (k == N) m |= 1 << 5;   // zeta+
   return m;
 }
-__device__ __forceinline__ void face_ab(int ls, int i, int j, int k, int& a, int& b) {
+
+// Own face-point index x = (ls-1)*nf + a + n*b of node (i,j,k) on local side ls.
+template <int N>
+__device__ __forceinline__ int face_point(int ls, int i, int j, int k) {
+  constexpr int n = N + 1, nf = n * n;
   switch (dir_of(ls)) {
-    case 0: a = j; b = k; break;
-    case 1: a = i; b = k; break;
-    default: a = i; b = j; break;
+    case 0: return (ls - 1) * nf + j + n * k;
+    case 1: return (ls - 1) * nf + i + n * k;
+    default: return (ls - 1) * nf + i + n * j;
   }
 }
-__device__ __forceinline__ double normal_sign(int ls) { return (ls == 3 || ls == 4 || ls == 6) ? 1.0 : -1.0; }
 
-// Side point (p,q) and slot (0 master, 1 slave) of own face (ls, a, b).
-struct FacePt {
-  size_t own;   // ((side*2 + ownslot)*nf + pq)
-  size_t nbr;   // ((side*2 + nbrslot)*nf + pq)
-  size_t sp;    // side*nf + pq, the side-array index (NormVec, SurfElem)
-  bool master;
-};
+// Side-array slot of own face point x of element e: ((side*2 + slot)*nf + pq), slot 0 master.
 template <int N>
-__device__ __forceinline__ FacePt face_pt(const DevMesh& m, int e, int ls, int a, int b) {
+__device__ __forceinline__ unsigned face_slot(const DevMesh& m, int e, int x) {
   constexpr int n = N + 1, nf = n * n;
+  const int ls = x / nf + 1, ab = x - (ls - 1) * nf;
   const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
   const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
-  const int* pq = m.v2s2 + 2 * ((a + n * b) + nf * (flip + 5 * (ls - 1)));
-  const size_t p = (size_t)(pq[0] + n * pq[1]);
-  const bool master = flip == 0;
-  FacePt f;
-  f.master = master;
-  f.own = ((size_t)side * 2 + (master ? 0 : 1)) * nf + p;
-  f.nbr = ((size_t)side * 2 + (master ? 1 : 0)) * nf + p;
-  f.sp = (size_t)side * nf + p;
-  return f;
+  const int* pq = m.v2s2 + 2 * (ab + nf * (flip + 5 * (ls - 1)));
+  return (unsigned)(((size_t)side * 2 + (flip ? 1 : 0)) * nf + pq[0] + n * pq[1]);
+}
+template <int N>
+__device__ __forceinline__ bool slot_is_master(unsigned own) { return ((own / (unsigned)((N + 1) * (N + 1))) & 1u) == 0; }
+template <int N>
+__device__ __forceinline__ unsigned slot_nbr(unsigned own) {
+  constexpr unsigned nf = (N + 1) * (N + 1);
+  return slot_is_master<N>(own) ? own + nf : own - nf;
+}
+template <int N>
+__device__ __forceinline__ size_t slot_side_point(unsigned own) {
+  constexpr unsigned nf = (N + 1) * (N + 1);
+  return (size_t)(own / (2 * nf)) * nf + own % nf;
 }
 
 // BR1 lifting at node t (strong form, non-conservative): D_T contractions of the lifting
-// variables held in LDS (sP: u,v,w,T as SoA), metric transform, surface corrections
-// 0.5*(U_nb - U_own).(n SurfElem)*L_hat at face nodes, times sJ (lifting_br1.t90:48-181).
+// variables held in LDS (sP: u,v,w,T as SoA rows of np), metric transform, surface corrections
+// 0.5*(U_nb - U_own).(n SurfElem)*L_hat at face nodes with the neighbour values staged per own
+// face point (sNb: u,v,w,T as SoA rows of NF), times sJ (lifting_br1.t90:48-181).
 template <int N, class RF>
-__device__ __forceinline__ void lift_node(const DevMesh& m, int e, int t, const double* sP, const double* sB,
-                                          const double* FU, const Gas& gas, const RF* Ja, RF sj, RF* g) {
-  constexpr int n = N + 1, np = n * n * n;
+__device__ __forceinline__ void lift_node(int t, const double* sP, const double* sB, const double* sNb,
+                                          const RF* Ja, RF sj, RF* g) {
+  constexpr int n = N + 1, np = n * n * n, NF = D<N>::NF;
   const BasisLayout B{n};
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
   const double* DT = sB + B.D_T();
@@ -123,14 +134,13 @@ __device__ __forceinline__ void lift_node(const DevMesh& m, int e, int t, const 
 #pragma unroll 1
     for (int ls = 1; ls <= 6; ++ls) {
       if (!(sides & (1 << (ls - 1)))) continue;
-      int a, b;
-      face_ab(ls, i, j, k, a, b);
-      const FacePt fp = face_pt<N>(m, e, ls, a, b);
-      const Prim q = cons_to_prim(FU + 5 * fp.nbr, gas);
+      const int x = face_point<N>(ls, i, j, k);
       const int dir = dir_of(ls);
       const RF s = (RF)normal_sign(ls) * RF(0.5) * (minus_side(ls) ? lhm : lhp);
       const RF nse[3] = {Ja[3 * dir] * s, Ja[3 * dir + 1] * s, Ja[3 * dir + 2] * s};
-      const RF dq[4] = {(RF)q.u - own[0], (RF)q.v - own[1], (RF)q.w - own[2], (RF)q.T - own[3]};
+      RF dq[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dq[v] = (RF)sNb[v * NF + x] - own[v];
 #pragma unroll
       for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -141,7 +151,7 @@ __device__ __forceinline__ void lift_node(const DevMesh& m, int e, int t, const 
   for (int c = 0; c < 12; ++c) g[c] *= sj;
 }
 
-template <int N, class RF>
+template <class RF>
 __device__ __forceinline__ void load_metrics(const DevMesh& m, size_t gid, RF* Ja, RF& sj) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -157,14 +167,22 @@ __device__ __forceinline__ void load_metrics(const DevMesh& m, size_t gid, RF* J
 template <int N, class RF>
 __global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restrict__ U, DevMesh m,
                                                        const double* __restrict__ FU, RF* __restrict__ FG, Gas gas) {
-  constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT;
-  __shared__ double sP[4 * np];
+  constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
+  __shared__ double sP[4 * np];    // u, v, w, T
+  __shared__ double sNb[4 * NF];   // neighbour u, v, w, T per own face point
+  __shared__ unsigned sIdx[NF];
   __shared__ double sB[3 * n * n + 4 * n];
   const int e = blockIdx.x, t = threadIdx.x;
   for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
   if (t < np) {
     const Prim q = cons_to_prim(U + 5 * ((size_t)np * e + t), gas);
     sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
+  }
+  for (int x = t; x < NF; x += NT) {
+    const unsigned own = face_slot<N>(m, e, x);
+    sIdx[x] = own;
+    const Prim q = cons_to_prim(FU + 5 * (size_t)slot_nbr<N>(own), gas);
+    sNb[x] = q.u; sNb[NF + x] = q.v; sNb[2 * NF + x] = q.w; sNb[3 * NF + x] = q.T;
   }
   __syncthreads();
   if (t >= np) return;
@@ -173,21 +191,18 @@ __global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restric
   if (!sides) return;
   const GasT<RF> gt(gas);
   RF Ja[9], sj;
-  load_metrics<N, RF>(m, (size_t)np * e + t, Ja, sj);
+  load_metrics<RF>(m, (size_t)np * e + t, Ja, sj);
   RF g[12];
-  lift_node<N, RF>(m, e, t, sP, sB, FU, gas, Ja, sj, g);
+  lift_node<N, RF>(t, sP, sB, sNb, Ja, sj, g);
   RF fx[4], fy[4], fz[4];
   diff_flux<RF>(gt, (RF)sP[t], (RF)sP[np + t], (RF)sP[2 * np + t], g, g + 4, g + 8, fx, fy, fz);
 #pragma unroll 1
   for (int ls = 1; ls <= 6; ++ls) {
     if (!(sides & (1 << (ls - 1)))) continue;
-    int a, b;
-    face_ab(ls, i, j, k, a, b);
-    const FacePt fp = face_pt<N>(m, e, ls, a, b);
     const int dir = dir_of(ls);
     const RF s = (RF)normal_sign(ls);
     const RF n0 = Ja[3 * dir] * s, n1 = Ja[3 * dir + 1] * s, n2 = Ja[3 * dir + 2] * s;
-    RF* dst = FG + 4 * fp.own;
+    RF* dst = FG + 4 * (size_t)sIdx[face_point<N>(ls, i, j, k)];
 #pragma unroll
     for (int c = 0; c < 4; ++c) dst[c] = fx[c] * n0 + fy[c] * n1 + fz[c] * n2;
   }
@@ -195,73 +210,64 @@ __global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restric
 
 // ---------------------------------------------------------------------------------------------
 // Kernel B: the rest of the stage.
+template <int N>
+struct LdsB {
+  static constexpr int np = D<N>::np, NF = D<N>::NF;
+  // sX holds the neighbour face states (7 rows of NF) and the surface contributions (5 rows of
+  // NF) until the face fluxes are gathered, then the direction loop's Ja^d (3 rows of np) and
+  // contravariant viscous fluxes (4 rows of np).
+  static constexpr int X = (7 * np > 12 * NF) ? 7 * np : 12 * NF;
+};
+
 template <int N, class RF>
-__global__ __launch_bounds__(D<N>::NT) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
+__global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
                                                      double* __restrict__ FUnext, const RF* __restrict__ FG,
                                                      Gas gas, int solver, double* Ut, RKStage st,
                                                      RF* __restrict__ Gout) {
-  constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT;
-  // sA slots: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T (E after the lifting)
-  __shared__ double sA[7 * np];
-  // per-direction Ja^d (3) + viscous contravariant flux (4), or the surface scratch
-  __shared__ double sX[(7 * np > 54 * nf) ? 7 * np : 54 * nf];
+  constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
+  // sA rows: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T, 7 E
+  __shared__ double sA[8 * np];
+  __shared__ double sX[LdsB<N>::X];
+  __shared__ unsigned sIdx[NF];
   __shared__ double sB[3 * n * n + 4 * n];
+  // neighbour rows in sX: 0 u, 1 v, 2 w, 3 T, 4 rho, 5 p, 6 E; surface rows follow
+  double* sNb = sX;
+  double* sSurf = sX + 7 * NF;
   const BasisLayout B{n};
   const GasT<RF> gt(gas);
   const int e = blockIdx.x, t = threadIdx.x;
   const bool act = t < np;
-  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
-  double Uown[5] = {0, 0, 0, 0, 0};
-  Prim q{};
   const size_t gid = (size_t)np * e + (act ? t : 0);
+  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
   if (act) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) Uown[v] = U[5 * gid + v];
-    q = cons_to_prim(Uown, gas);
-    sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (Uown[4] + q.p) / Uown[0];
-    sA[3 * np + t] = q.u; sA[4 * np + t] = q.v; sA[5 * np + t] = q.w; sA[6 * np + t] = q.T;
+    const double* c = U + 5 * gid;
+    const Prim q = cons_to_prim(c, gas);
+    const double E = c[4];
+    sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (E + q.p) / q.rho;
+    sA[3 * np + t] = q.u; sA[4 * np + t] = q.v; sA[5 * np + t] = q.w; sA[6 * np + t] = q.T; sA[7 * np + t] = E;
+  }
+  for (int x = t; x < NF; x += NT) {
+    const unsigned own = face_slot<N>(m, e, x);
+    sIdx[x] = own;
+    const double* c = FU + 5 * (size_t)slot_nbr<N>(own);
+    const Prim q = cons_to_prim(c, gas);
+    sNb[x] = q.u; sNb[NF + x] = q.v; sNb[2 * NF + x] = q.w; sNb[3 * NF + x] = q.T;
+    sNb[4 * NF + x] = q.rho; sNb[5 * NF + x] = q.p; sNb[6 * NF + x] = c[4];
   }
   __syncthreads();
-  const int i = t % n, j = (t / n) % n, k = t / (n * n);
-  RF Ja[9], sj;
-  RF fx[4] = {0, 0, 0, 0}, fy[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0};
-  if (act) {
-    load_metrics<N, RF>(m, gid, Ja, sj);
-    RF g[12];
-    lift_node<N, RF>(m, e, t, sA + 3 * np, sB, FU, gas, Ja, sj, g);  // u,v,w,T
-    diff_flux<RF>(gt, (RF)q.u, (RF)q.v, (RF)q.w, g, g + 4, g + 8, fx, fy, fz);
-    if (Gout) {  // (np, 12, nElems) like the four-kernel path
-#pragma unroll
-      for (int c = 0; c < 12; ++c) Gout[t + (size_t)np * (c + 12 * (size_t)e)] = g[c];
-    }
-  }
-  __syncthreads();  // T (sA slot 6) and all lifting reads are done
-  const int sides = act ? node_sides<N>(i, j, k) : 0;
-  double* sGE = sX;              // [6*nf][4]: own viscous normal flux per own face point
-  double* sSurf = sX + 24 * nf;  // [6*nf][5]: surface contribution per own face point
-  if (act) {
-    sA[6 * np + t] = Uown[4];  // E for the face states
-#pragma unroll 1
-    for (int ls = 1; ls <= 6; ++ls) {
-      if (!(sides & (1 << (ls - 1)))) continue;
-      int a, b;
-      face_ab(ls, i, j, k, a, b);
-      const int dir = dir_of(ls);
-      const RF s = (RF)normal_sign(ls);
-      const RF n0 = Ja[3 * dir] * s, n1 = Ja[3 * dir + 1] * s, n2 = Ja[3 * dir + 2] * s;
-      double* dst = sGE + 4 * ((ls - 1) * nf + a + n * b);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) dst[c] = (double)(fx[c] * n0 + fy[c] * n1 + fz[c] * n2);
-    }
-  }
-  __syncthreads();
-  // surface phase: one lane per own face point
-  for (int x = t; x < 6 * nf; x += NT) {
+
+  // Surface phase, one lane per own face point: Riemann flux with the neighbour's state and the
+  // neighbour's half of the viscous normal flux; the own half is added by the node itself below.
+  for (int x = t; x < NF; x += NT) {
     const int ls = x / nf + 1, ab = x - (ls - 1) * nf;
-    const int a = ab % n, b = ab / n;
-    const int id = vidx<N>(ls, a, b);
-    const FacePt fp = face_pt<N>(m, e, ls, a, b);
-    const double ms = fp.master ? 1.0 : -1.0;
+    const int id = vidx<N>(ls, ab % n, ab / n);
+    const unsigned own = sIdx[x];
+    const bool master = slot_is_master<N>(own);
+    const unsigned nb = slot_nbr<N>(own);
+    const double ms = master ? 1.0 : -1.0;
+    RF gn[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gn[c] = FG[4 * (size_t)nb + c];
     double se;
     RF nv[3];  // master normal
     if (m.ownFaceMetric) {
@@ -274,8 +280,9 @@ __global__ __launch_bounds__(D<N>::NT) void k_dg_fused(const double* U, DevMesh 
       nv[0] = (RF)(ms * ns0 / se); nv[1] = (RF)(ms * ns1 / se); nv[2] = (RF)(ms * ns2 / se);
     } else {
       // the side's normal and surface element, shared by master and slave as in FLEXI
-      se = m.SurfElem[fp.sp];
-      nv[0] = (RF)m.NormVec[3 * fp.sp]; nv[1] = (RF)m.NormVec[3 * fp.sp + 1]; nv[2] = (RF)m.NormVec[3 * fp.sp + 2];
+      const size_t sp = slot_side_point<N>(own);
+      se = m.SurfElem[sp];
+      nv[0] = (RF)m.NormVec[3 * sp]; nv[1] = (RF)m.NormVec[3 * sp + 1]; nv[2] = (RF)m.NormVec[3 * sp + 2];
     }
     // tangential basis: Gram-Schmidt of the coordinate axis least aligned with n
     RF t1[3], t2[3];
@@ -291,63 +298,91 @@ __global__ __launch_bounds__(D<N>::NT) void k_dg_fused(const double* U, DevMesh 
       t2[1] = nv[2] * t1[0] - nv[0] * t1[2];
       t2[2] = nv[0] * t1[1] - nv[1] * t1[0];
     }
-    const Prim qn = cons_to_prim(FU + 5 * fp.nbr, gas);
-    const double Enb = FU[5 * fp.nbr + 4];
-    const RotState<RF> So = rotate<RF>((RF)sA[id], (RF)sA[6 * np + id], (RF)sA[np + id], (RF)sA[3 * np + id],
+    const RotState<RF> So = rotate<RF>((RF)sA[id], (RF)sA[7 * np + id], (RF)sA[np + id], (RF)sA[3 * np + id],
                                        (RF)sA[4 * np + id], (RF)sA[5 * np + id], nv, t1, t2);
-    const RotState<RF> Sn = rotate<RF>((RF)qn.rho, (RF)Enb, (RF)qn.p, (RF)qn.u, (RF)qn.v, (RF)qn.w, nv, t1, t2);
+    const RotState<RF> Sn = rotate<RF>((RF)sNb[4 * NF + x], (RF)sNb[6 * NF + x], (RF)sNb[5 * NF + x], (RF)sNb[x],
+                                       (RF)sNb[NF + x], (RF)sNb[2 * NF + x], nv, t1, t2);
     RF F[5];
-    if (fp.master) riemann_rot<RF>(gt, solver, true, So, Sn, F);
+    if (master) riemann_rot<RF>(gt, solver, true, So, Sn, F);
     else riemann_rot<RF>(gt, solver, true, Sn, So, F);
-    RF Fo[5];
-    Fo[0] = F[0];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Fo[1 + c] = nv[c] * F[1] + t1[c] * F[2] + t2[c] * F[3];
-    Fo[4] = F[4];
-    const double* ge = sGE + 4 * x;
-    const RF* gn = FG + 4 * fp.nbr;
     const double lh = minus_side(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N];
-    double* dst = sSurf + 5 * x;
-    dst[0] = ms * (double)Fo[0] * se * lh;
+    sSurf[x] = ms * (double)F[0] * se * lh;
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      dst[1 + c] = (ms * (double)Fo[1 + c] * se + 0.5 * (ge[c] - (double)gn[c])) * lh;
+    for (int c = 0; c < 3; ++c) {
+      const RF Fc = nv[c] * F[1] + t1[c] * F[2] + t2[c] * F[3];
+      sSurf[(1 + c) * NF + x] = (ms * (double)Fc * se - 0.5 * (double)gn[c]) * lh;
+    }
+    sSurf[4 * NF + x] = (ms * (double)F[4] * se - 0.5 * (double)gn[3]) * lh;
   }
-  __syncthreads();
+
+  // Lifting and viscous fluxes at every node; the own half of the viscous normal face fluxes.
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  const int sides = act ? node_sides<N>(i, j, k) : 0;
   RF acc[5] = {0, 0, 0, 0, 0};
+  RF fv[12];  // contravariant viscous flux Ja^d . (f,g,h)_v, d-major
+  if (act) {
+    RF Ja[9], sj;
+    load_metrics<RF>(m, gid, Ja, sj);
+    RF g[12];
+    lift_node<N, RF>(t, sA + 3 * np, sB, sNb, Ja, sj, g);  // u,v,w,T rows
+    if (Gout) {  // (np, 12, nElems) like the four-kernel path
+#pragma unroll
+      for (int c = 0; c < 12; ++c) Gout[t + (size_t)np * (c + 12 * (size_t)e)] = g[c];
+    }
+    RF fx[4], fy[4], fz[4];
+    diff_flux<RF>(gt, (RF)sA[3 * np + t], (RF)sA[4 * np + t], (RF)sA[5 * np + t], g, g + 4, g + 8, fx, fy, fz);
+#pragma unroll 1
+    for (int ls = 1; ls <= 6; ++ls) {
+      if (!(sides & (1 << (ls - 1)))) continue;
+      const int dir = dir_of(ls);
+      const RF s = (RF)normal_sign(ls);
+      const RF n0 = Ja[3 * dir] * s, n1 = Ja[3 * dir + 1] * s, n2 = Ja[3 * dir + 2] * s;
+      const RF lh = (RF)(minus_side(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[1 + c] += RF(0.5) * (fx[c] * n0 + fy[c] * n1 + fz[c] * n2) * lh;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) fv[4 * d + c] = Ja[3 * d] * fx[c] + Ja[3 * d + 1] * fy[c] + Ja[3 * d + 2] * fz[c];
+  }
+  __syncthreads();  // surface contributions complete
   if (sides) {
 #pragma unroll 1
     for (int ls = 1; ls <= 6; ++ls) {
       if (!(sides & (1 << (ls - 1)))) continue;
-      int a, b;
-      face_ab(ls, i, j, k, a, b);
-      const double* src = sSurf + 5 * ((ls - 1) * nf + a + n * b);
+      const int x = face_point<N>(ls, i, j, k);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) acc[v] += (RF)src[v];
+      for (int v = 0; v < 5; ++v) acc[v] += (RF)sSurf[v * NF + x];
     }
   }
   __syncthreads();  // sX is reused by the direction loop
+
   // split-form volume integral with the viscous weak-form part (volint.f90:211-371)
   const double* DV = sB + B.DVolSurf();
   const double* DH = sB + B.D_Hat_T();
-  const RF rho = (RF)q.rho, uu = (RF)q.u, vv = (RF)q.v, ww = (RF)q.w, pp = (RF)q.p;
-  const RF HH = act ? (RF)sA[2 * np + t] : RF(0);
+  RF rho = 0, uu = 0, vv = 0, ww = 0, pp = 0, HH = 0;
+  if (act) {
+    rho = (RF)sA[t]; pp = (RF)sA[np + t]; HH = (RF)sA[2 * np + t];
+    uu = (RF)sA[3 * np + t]; vv = (RF)sA[4 * np + t]; ww = (RF)sA[5 * np + t];
+  }
   double* sM = sX;
   double* sF = sX + 3 * np;
-#pragma unroll 1
-  for (int d = 0; d < 3; ++d) {
-    if (act) {
-      const RF* Jd = Ja + 3 * d;
-      sM[t] = (double)Jd[0]; sM[np + t] = (double)Jd[1]; sM[2 * np + t] = (double)Jd[2];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sF[c * np + t] = (double)(Jd[0] * fx[c] + Jd[1] * fy[c] + Jd[2] * fz[c]);
+  for (int d = 0; d < 3; ++d) {
+    RF m0 = 0, m1 = 0, m2 = 0;
+    if (act) {
+      const double* Jg = (d == 0 ? m.Mf : (d == 1 ? m.Mg : m.Mh)) + 3 * gid;
+      m0 = (RF)Jg[0]; m1 = (RF)Jg[1]; m2 = (RF)Jg[2];
+      sM[t] = (double)m0; sM[np + t] = (double)m1; sM[2 * np + t] = (double)m2;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sF[c * np + t] = (double)fv[4 * d + c];
     }
     __syncthreads();
     if (act) {
       const int me = d == 0 ? i : (d == 1 ? j : k);
       const int stride = d == 0 ? 1 : (d == 1 ? n : n * n);
       const int base = t - me * stride;
-      const RF m0 = Ja[3 * d], m1 = Ja[3 * d + 1], m2 = Ja[3 * d + 2];
 #pragma unroll
       for (int l = 0; l <= N; ++l) {
         const int id = base + l * stride;
@@ -388,17 +423,15 @@ __global__ __launch_bounds__(D<N>::NT) void k_dg_fused(const double* U, DevMesh 
   for (int v = 0; v < 5; ++v) {
     const double r = (st.mode == 1) ? ut[v] : st.R[o + v] * st.minusA + ut[v];
     st.R[o + v] = r;
-    un[v] = Uown[v] + r * st.b_dt;
-    st.U[o + v] = un[v];
+    un[v] = U[o + v] + r * st.b_dt;
   }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) st.U[o + v] = un[v];
   // face values of the new state for the next stage
 #pragma unroll 1
   for (int ls = 1; ls <= 6; ++ls) {
     if (!(sides & (1 << (ls - 1)))) continue;
-    int a, b;
-    face_ab(ls, i, j, k, a, b);
-    const FacePt fp = face_pt<N>(m, e, ls, a, b);
-    double* dst = FUnext + 5 * fp.own;
+    double* dst = FUnext + 5 * (size_t)sIdx[face_point<N>(ls, i, j, k)];
 #pragma unroll
     for (int v = 0; v < 5; ++v) dst[v] = un[v];
   }
@@ -407,8 +440,11 @@ __global__ __launch_bounds__(D<N>::NT) void k_dg_fused(const double* U, DevMesh 
 // Face states of U into FU (all six faces of every element).
 template <int N>
 __global__ __launch_bounds__(D<N>::NT) void k_faces_of(const double* __restrict__ U, DevMesh m, double* __restrict__ FU) {
-  constexpr int n = D<N>::n, np = D<N>::np;
+  constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
+  __shared__ unsigned sIdx[NF];
   const int e = blockIdx.x, t = threadIdx.x;
+  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot<N>(m, e, x);
+  __syncthreads();
   if (t >= np) return;
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
   const int sides = node_sides<N>(i, j, k);
@@ -420,11 +456,9 @@ __global__ __launch_bounds__(D<N>::NT) void k_faces_of(const double* __restrict_
 #pragma unroll 1
   for (int ls = 1; ls <= 6; ++ls) {
     if (!(sides & (1 << (ls - 1)))) continue;
-    int a, b;
-    face_ab(ls, i, j, k, a, b);
-    const FacePt fp = face_pt<N>(m, e, ls, a, b);
+    double* dst = FU + 5 * (size_t)sIdx[face_point<N>(ls, i, j, k)];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) FU[5 * fp.own + v] = u[v];
+    for (int v = 0; v < 5; ++v) dst[v] = u[v];
   }
 }
 
@@ -473,6 +507,9 @@ hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas
   }
 }
 
-bool fused_supported(int N, int nodeType, int split) { return nodeType == 2 && split && N >= 2 && N <= 7; }
+// Face slots are 32-bit: nSides * 2 * (N+1)^2 must stay below 2^32.
+bool fused_supported(int N, int nodeType, int split, long long nSides) {
+  return nodeType == 2 && split && N >= 2 && N <= 7 && nSides * 2LL * (N + 1) * (N + 1) < (1LL << 32);
+}
 
 }  // namespace fxg

@@ -90,7 +90,7 @@ struct FusedWork {
   double* FU[2];  // face states, 5 per side point and slot, double-buffered across stages
   void* FG;       // viscous normal face fluxes f_v.(n SurfElem), 4 per side point and slot
 };
-bool fused_supported(int N, int nodeType, int split);
+bool fused_supported(int N, int nodeType, int split, long long nSides);
 // One stage of the fused path. FU[cur] must hold the face states of U unless faces_fresh is
 // false, in which case they are rebuilt first. In RK modes the new state's faces go to FU[1-cur].
 // G (optional, (np,12,nElems) of the gradient type) receives the lifted gradients.
