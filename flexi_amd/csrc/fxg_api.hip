@@ -257,7 +257,7 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   if (c->fused) {
     char* FG;
     if (alloc_owned(c, &c->fw.FU[0], 10 * face) || alloc_owned(c, &c->fw.FU[1], 10 * face) ||
-        alloc_owned(c, &FG, 8 * face * gsz))
+        alloc_owned(c, &FG, 8 * face * gsz) || alloc_owned(c, &c->fw.Jsoa, 10 * vol))
       return 1;
     c->fw.FG = FG;
   }
@@ -272,6 +272,7 @@ int finish_setup(fxg_ctx* c) {
   const double KappasPr = c->p.kappa / c->p.Pr;
   const double kmax = std::fmax(4.0 / 3.0, KappasPr);  // KAPPASPR_MAX_TIMESTEP_H, eos.h
   HIPCHECK(fxg::launch_metrics_adv(c->m, kmax, c->adv, c->visc, c->stream));
+  if (c->fused) HIPCHECK(fxg::launch_pack_metrics(c->m, c->fw.Jsoa, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
 }

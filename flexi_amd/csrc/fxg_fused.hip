@@ -162,22 +162,32 @@ __device__ __forceinline__ void load_metrics(const DevMesh& m, size_t gid, RF* J
   sj = (RF)m.sJ[gid];
 }
 
+// Metric terms of node t from the element's block of the packed array (rows of np: Ja^1 x,y,z,
+// Ja^2 x,y,z, Ja^3 x,y,z, sJ), so that consecutive lanes load consecutive doubles.
+template <int N, class RF>
+__device__ __forceinline__ void load_metrics(const double* Je, int t, RF* Ja, RF& sj) {
+  constexpr int np = D<N>::np;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) Ja[r] = (RF)Je[r * np + t];
+  sj = (RF)Je[9 * np + t];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Kernel A: viscous normal fluxes of the element's faces.
 template <int N, class RF>
 __global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restrict__ U, DevMesh m,
-                                                       const double* __restrict__ FU, RF* __restrict__ FG, Gas gas) {
+                                                       const double* __restrict__ Jsoa, const double* __restrict__ FU,
+                                                       RF* __restrict__ FG, Gas gas) {
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
+  __shared__ double sRaw[5 * np];  // the element's U as stored, loaded with contiguous lanes
   __shared__ double sP[4 * np];    // u, v, w, T
   __shared__ double sNb[4 * NF];   // neighbour u, v, w, T per own face point
   __shared__ unsigned sIdx[NF];
   __shared__ double sB[3 * n * n + 4 * n];
   const int e = blockIdx.x, t = threadIdx.x;
+  const double* Ue = U + (size_t)5 * np * e;
+  for (int x = t; x < 5 * np; x += NT) sRaw[x] = Ue[x];
   for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
-  if (t < np) {
-    const Prim q = cons_to_prim(U + 5 * ((size_t)np * e + t), gas);
-    sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
-  }
   for (int x = t; x < NF; x += NT) {
     const unsigned own = face_slot<N>(m, e, x);
     sIdx[x] = own;
@@ -185,13 +195,19 @@ __global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restric
     sNb[x] = q.u; sNb[NF + x] = q.v; sNb[2 * NF + x] = q.w; sNb[3 * NF + x] = q.T;
   }
   __syncthreads();
+  if (t < np) {
+    const Prim q = cons_to_prim(sRaw + 5 * t, gas);
+    sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
+  }
+  __syncthreads();
   if (t >= np) return;
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
   const int sides = node_sides<N>(i, j, k);
   if (!sides) return;
   const GasT<RF> gt(gas);
+  const double* Je = Jsoa + (size_t)10 * np * e;
   RF Ja[9], sj;
-  load_metrics<RF>(m, (size_t)np * e + t, Ja, sj);
+  load_metrics<N, RF>(Je, t, Ja, sj);
   RF g[12];
   lift_node<N, RF>(t, sP, sB, sNb, Ja, sj, g);
   RF fx[4], fy[4], fz[4];
@@ -219,11 +235,12 @@ struct LdsB {
   static constexpr int X = (7 * np > 12 * NF) ? 7 * np : 12 * NF;
 };
 
-template <int N, class RF>
-__global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
+// The Riemann solver is a template parameter: each instance then carries only its own solver's
+// code, which keeps the kernel within 128 VGPRs (two workgroups per CU at N=7) without spills.
+template <int N, class RF, int SOLVER>
+__global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
                                                      double* __restrict__ FUnext, const RF* __restrict__ FG,
-                                                     Gas gas, int solver, double* Ut, RKStage st,
-                                                     RF* __restrict__ Gout) {
+                                                     Gas gas, double* Ut, RKStage st, RF* __restrict__ Gout) {
   constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
   // sA rows: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T, 7 E
   __shared__ double sA[8 * np];
@@ -303,8 +320,8 @@ __global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) v
     const RotState<RF> Sn = rotate<RF>((RF)sNb[4 * NF + x], (RF)sNb[6 * NF + x], (RF)sNb[5 * NF + x], (RF)sNb[x],
                                        (RF)sNb[NF + x], (RF)sNb[2 * NF + x], nv, t1, t2);
     RF F[5];
-    if (master) riemann_rot<RF>(gt, solver, true, So, Sn, F);
-    else riemann_rot<RF>(gt, solver, true, Sn, So, F);
+    if (master) riemann_rot<RF>(gt, SOLVER, true, So, Sn, F);
+    else riemann_rot<RF>(gt, SOLVER, true, Sn, So, F);
     const double lh = minus_side(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N];
     sSurf[x] = ms * (double)F[0] * se * lh;
 #pragma unroll
@@ -481,17 +498,46 @@ hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int rie
   }
   {
     Scope sc(tm, K_VISC_FACES, s);
-    hipLaunchKernelGGL((k_visc_faces<N, RF>), dim3(K), dim3(NT), 0, s, U, m, w.FU[cur], (RF*)w.FG, gas);
+    hipLaunchKernelGGL((k_visc_faces<N, RF>), dim3(K), dim3(NT), 0, s, U, m, w.Jsoa, w.FU[cur], (RF*)w.FG, gas);
   }
   {
     Scope sc(tm, K_DG_FUSED, s);
-    hipLaunchKernelGGL((k_dg_fused<N, RF>), dim3(K), dim3(NT), 0, s, U, m, w.FU[cur], w.FU[1 - cur], (const RF*)w.FG,
-                       gas, riemann, Ut, st, (RF*)G);
+    const double* FUc = w.FU[cur];
+    double* FUn = w.FU[1 - cur];
+    const RF* FG = (const RF*)w.FG;
+    if (riemann == 1)
+      hipLaunchKernelGGL((k_dg_fused<N, RF, 1>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, (RF*)G);
+    else if (riemann == 3)
+      hipLaunchKernelGGL((k_dg_fused<N, RF, 3>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, (RF*)G);
+    else
+      hipLaunchKernelGGL((k_dg_fused<N, RF, 33>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, (RF*)G);
   }
   return hipGetLastError();
 }
 
+// Packs the FLEXI-layout metric terms (3,np,K) x 3 and sJ (np,K) into rows of np per element.
+__global__ __launch_bounds__(256) void k_pack_metrics(DevMesh m, int np, size_t nNodes, double* __restrict__ J) {
+  const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nNodes) return;
+  const size_t e = x / np, t = x - e * np;
+  double* Je = J + 10 * np * e + t;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    Je[(0 + c) * np] = m.Mf[3 * x + c];
+    Je[(3 + c) * np] = m.Mg[3 * x + c];
+    Je[(6 + c) * np] = m.Mh[3 * x + c];
+  }
+  Je[9 * np] = m.sJ[x];
+}
+
 }  // namespace
+
+hipError_t launch_pack_metrics(const DevMesh& m, double* Jsoa, hipStream_t s) {
+  const int np = (m.N + 1) * (m.N + 1) * (m.N + 1);
+  const size_t nNodes = (size_t)np * m.nElems;
+  hipLaunchKernelGGL(k_pack_metrics, dim3((unsigned)((nNodes + 255) / 256)), dim3(256), 0, s, m, np, nNodes, Jsoa);
+  return hipGetLastError();
+}
 
 hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, int fp32,
                             const double* U, double* Ut, const RKStage& st, bool faces_fresh, int cur, void* G,
