@@ -154,6 +154,9 @@ struct fxg_ctx {
   bool fused = false;
   fxg::FusedWork fw{};
   int cur = 0;
+  // discrete adjoint scratch, allocated on first use
+  fxg::AdjointWork aw{};
+  double* adjTmp = nullptr;
 };
 
 namespace {
@@ -301,6 +304,13 @@ int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStre
   return 0;
 }
 
+// One LSERK stage (st = 0 .. nStages-1) of TimeStepByLSERKW2 on (U, R).
+int lserk_stage(fxg_ctx* c, double dt, int st, const double* RKA, const double* RKb, double* U, double* R,
+                hipStream_t s) {
+  fxg::RKStage stage{st == 0 ? 1 : 2, st == 0 ? 0.0 : -RKA[st - 1], RKb[st] * dt, U, R};
+  return rhs(c, U, nullptr, stage, s, st > 0);
+}
+
 int lserk(fxg_ctx* c, double dt, int nStages, const double* RKA, const double* RKb, double* U, double* R,
           hipStream_t s) {
   if (nStages < 1) return fail("nStages must be positive");
@@ -311,6 +321,38 @@ int lserk(fxg_ctx* c, double dt, int nStages, const double* RKA, const double* R
   }
   return 0;
 }
+
+int check_rk(int nStages, const double* RKA, const double* RKb) {
+  if (nStages < 1) return fail("nStages must be positive");
+  if (!RKb || (nStages > 1 && !RKA)) return fail("RK coefficient arrays are NULL");
+  return 0;
+}
+
+int ensure_adjoint(fxg_ctx* c) {
+  if (!c->fused || c->p.precision != FXG_PREC_FP64)
+    return fail("the discrete adjoint is built for the FP64 Gauss-Lobatto split form (N 2..7)");
+  if (c->aw.FUa) return 0;
+  const size_t face = (size_t)c->nf * c->S, vol = (size_t)c->np * c->K;
+  if (alloc_owned(c, &c->aw.FUa, 10 * face) || alloc_owned(c, &c->aw.XU, 10 * face) ||
+      alloc_owned(c, &c->aw.Xg, 8 * face) || alloc_owned(c, &c->aw.Xq, 8 * face) ||
+      alloc_owned(c, &c->adjTmp, 5 * vol))
+    return 1;
+  return 0;
+}
+
+// Device buffers of one adjoint run, released on every exit path.
+struct DevBuffers {
+  std::vector<void*> p;
+  ~DevBuffers() {
+    for (void* x : p) (void)hipFree(x);
+  }
+  double* get(size_t count) {
+    double* d = nullptr;
+    if (hipMalloc((void**)&d, count * sizeof(double)) != hipSuccess) return nullptr;
+    p.push_back(d);
+    return d;
+  }
+};
 
 }  // namespace
 
@@ -505,6 +547,79 @@ int fxg_profile_read(fxg_ctx* c, int kernel, double* total_ms, int* launches) {
   }
   *total_ms = tot;
   *launches = cnt;
+  return 0;
+}
+
+int fxg_rhs_vjp_dev(fxg_ctx* c, double t, const double* U, const double* lam, double* out, void* stream) {
+  (void)t;
+  if (!c || !U || !lam || !out) return fail("NULL argument");
+  if (ensure_adjoint(c)) return 1;
+  HIPCHECK(fxg::launch_rhs_vjp(c->m, c->aw, c->gas, c->p.riemann, U, lam, out,
+                               stream ? (hipStream_t)stream : c->stream));
+  return 0;
+}
+
+int fxg_forward_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, const double* RKA, const double* RKb,
+                        const double* RKc, double* U, void* stream) {
+  (void)RKc;
+  if (!c || !dt || !U) return fail("NULL argument");
+  if (nSteps < 0) return fail("nSteps must not be negative");
+  if (check_rk(nStages, RKA, RKb)) return 1;
+  const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  for (int it = 0; it < nSteps; ++it)
+    if (lserk(c, dt[it], nStages, RKA, RKb, U, c->R, s)) return 1;
+  return 0;
+}
+
+int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, const double* RKA, const double* RKb,
+                        const double* RKc, const double* U0, double* ubar, int checkpointEvery, void* stream) {
+  (void)RKc;
+  if (!c || !dt || !U0 || !ubar) return fail("NULL argument");
+  if (nSteps < 0) return fail("nSteps must not be negative");
+  if (checkpointEvery < 1) return fail("checkpointEvery must be positive");
+  if (check_rk(nStages, RKA, RKb)) return 1;
+  if (ensure_adjoint(c)) return 1;
+  if (nSteps == 0) return 0;
+  const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const size_t vol5 = (size_t)5 * c->np * c->K, bytes = vol5 * sizeof(double);
+  const int every = checkpointEvery < nSteps ? checkpointEvery : nSteps;
+  const int nck = (nSteps + every - 1) / every;
+  DevBuffers buf;
+  double* ck = buf.get((size_t)nck * vol5);
+  double* seg = buf.get((size_t)every * nStages * vol5);
+  double* Uw = buf.get(vol5);
+  double* R = buf.get(vol5);
+  double* rbar = buf.get(vol5);
+  if (!ck || !seg || !Uw || !R || !rbar) return fail("fxg_adjoint_n_steps: out of device memory for checkpoints");
+  // forward sweep with a checkpoint of U every `every` steps
+  HIPCHECK(hipMemcpyAsync(Uw, U0, bytes, hipMemcpyDeviceToDevice, s));
+  for (int it = 0; it < nSteps; ++it) {
+    if (it % every == 0) HIPCHECK(hipMemcpyAsync(ck + (size_t)(it / every) * vol5, Uw, bytes, hipMemcpyDeviceToDevice, s));
+    if (lserk(c, dt[it], nStages, RKA, RKb, Uw, R, s)) return 1;
+  }
+  // reverse sweep, one segment at a time: recompute the stage states of the segment from its
+  // checkpoint, then run the adjoint of each stage backwards (R_s = Ut_s - RKA_s R_{s-1},
+  // U_s = U_{s-1} + RKb_s dt R_s):  rbar += RKb_s dt ubar;  ubar += (dUt/dU)^T rbar;
+  // rbar *= -RKA_s (0 for the first stage)
+  HIPCHECK(hipMemsetAsync(rbar, 0, bytes, s));
+  for (int sg = nck - 1; sg >= 0; --sg) {
+    const int s0 = sg * every, s1 = (s0 + every < nSteps) ? s0 + every : nSteps;
+    HIPCHECK(hipMemcpyAsync(Uw, ck + (size_t)sg * vol5, bytes, hipMemcpyDeviceToDevice, s));
+    for (int it = s0; it < s1; ++it)
+      for (int st = 0; st < nStages; ++st) {
+        HIPCHECK(hipMemcpyAsync(seg + ((size_t)(it - s0) * nStages + st) * vol5, Uw, bytes, hipMemcpyDeviceToDevice, s));
+        if (lserk_stage(c, dt[it], st, RKA, RKb, Uw, R, s)) return 1;
+      }
+    for (int it = s1 - 1; it >= s0; --it)
+      for (int st = nStages - 1; st >= 0; --st) {
+        const double* Us = seg + ((size_t)(it - s0) * nStages + st) * vol5;
+        HIPCHECK(fxg::launch_axpy(vol5, RKb[st] * dt[it], ubar, rbar, s));
+        HIPCHECK(fxg::launch_rhs_vjp(c->m, c->aw, c->gas, c->p.riemann, Us, rbar, c->adjTmp, s));
+        HIPCHECK(fxg::launch_axpy(vol5, 1.0, c->adjTmp, ubar, s));
+        HIPCHECK(fxg::launch_scale(vol5, st == 0 ? 0.0 : -RKA[st - 1], rbar, s));
+      }
+  }
+  HIPCHECK(hipStreamSynchronize(s));
   return 0;
 }
 

@@ -25,152 +25,10 @@
 // round-off, and multiplied by small quantities there). Each element gets FLEXI's contributions:
 // master +Flux, slave -Flux (src/dg/surfint.t90:471-616) with
 // Flux = [F*(U_m,U_s;n_m) + (f_m+f_s).n_m/2]*SurfElem (src/dg/fillflux.f90:40-215).
-#include "fxg_kernels.h"
+#include "fxg_elem.h"
 
 namespace fxg {
 namespace {
-
-template <int N>
-struct D {
-  static constexpr int n = N + 1, nf = n * n, np = n * n * n, NT = ((np + 63) / 64) * 64;
-  static constexpr int NF = 6 * nf;  // own face points of an element
-};
-
-__device__ __forceinline__ bool minus_side(int ls) { return ls == 5 || ls == 2 || ls == 1; }
-__device__ __forceinline__ int dir_of(int ls) { return (ls == 5 || ls == 3) ? 0 : (ls == 2 || ls == 4) ? 1 : 2; }
-__device__ __forceinline__ double normal_sign(int ls) { return (ls == 3 || ls == 4 || ls == 6) ? 1.0 : -1.0; }
-
-// Volume node of own face point (ls, a, b).
-template <int N>
-__device__ __forceinline__ int vidx(int ls, int a, int b) {
-  constexpr int n = N + 1;
-  const int l = minus_side(ls) ? 0 : N;
-  switch (dir_of(ls)) {
-    case 0: return l + n * (a + n * b);
-    case 1: return a + n * (l + n * b);
-    default: return a + n * (b + n * l);
-  }
-}
-
-// Local sides a node lies on (bit ls-1 set).
-template <int N>
-__device__ __forceinline__ int node_sides(int i, int j, int k) {
-  int m = 0;
-  if (k == 0) m |= 1 << 0;   // zeta-
-  if (j == 0) m |= 1 << 1;   // eta-
-  if (i == N) m |= 1 << 2;   // xi+
-  if (j == N) m |= 1 << 3;   // eta+
-  if (i == 0) m |= 1 << 4;   // xi-
-  if (k == N) m |= 1 << 5;   // zeta+
-  return m;
-}
-
-// Own face-point index x = (ls-1)*nf + a + n*b of node (i,j,k) on local side ls.
-template <int N>
-__device__ __forceinline__ int face_point(int ls, int i, int j, int k) {
-  constexpr int n = N + 1, nf = n * n;
-  switch (dir_of(ls)) {
-    case 0: return (ls - 1) * nf + j + n * k;
-    case 1: return (ls - 1) * nf + i + n * k;
-    default: return (ls - 1) * nf + i + n * j;
-  }
-}
-
-// Side-array slot of own face point x of element e: ((side*2 + slot)*nf + pq), slot 0 master.
-template <int N>
-__device__ __forceinline__ unsigned face_slot(const DevMesh& m, int e, int x) {
-  constexpr int n = N + 1, nf = n * n;
-  const int ls = x / nf + 1, ab = x - (ls - 1) * nf;
-  const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
-  const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
-  const int* pq = m.v2s2 + 2 * (ab + nf * (flip + 5 * (ls - 1)));
-  return (unsigned)(((size_t)side * 2 + (flip ? 1 : 0)) * nf + pq[0] + n * pq[1]);
-}
-template <int N>
-__device__ __forceinline__ bool slot_is_master(unsigned own) { return ((own / (unsigned)((N + 1) * (N + 1))) & 1u) == 0; }
-template <int N>
-__device__ __forceinline__ unsigned slot_nbr(unsigned own) {
-  constexpr unsigned nf = (N + 1) * (N + 1);
-  return slot_is_master<N>(own) ? own + nf : own - nf;
-}
-template <int N>
-__device__ __forceinline__ size_t slot_side_point(unsigned own) {
-  constexpr unsigned nf = (N + 1) * (N + 1);
-  return (size_t)(own / (2 * nf)) * nf + own % nf;
-}
-
-// BR1 lifting at node t (strong form, non-conservative): D_T contractions of the lifting
-// variables held in LDS (sP: u,v,w,T as SoA rows of np), metric transform, surface corrections
-// 0.5*(U_nb - U_own).(n SurfElem)*L_hat at face nodes with the neighbour values staged per own
-// face point (sNb: u,v,w,T as SoA rows of NF), times sJ (lifting_br1.t90:48-181).
-template <int N, class RF>
-__device__ __forceinline__ void lift_node(int t, const double* sP, const double* sB, const double* sNb,
-                                          const RF* Ja, RF sj, RF* g) {
-  constexpr int n = N + 1, np = n * n * n, NF = D<N>::NF;
-  const BasisLayout B{n};
-  const int i = t % n, j = (t / n) % n, k = t / (n * n);
-  const double* DT = sB + B.D_T();
-  RF gxi[4], geta[4], gzeta[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) gxi[v] = geta[v] = gzeta[v] = RF(0);
-#pragma unroll
-  for (int l = 0; l <= N; ++l) {
-    const RF dxi = (RF)DT[l + n * i], deta = (RF)DT[l + n * j], dzeta = (RF)DT[l + n * k];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      gxi[v] += dxi * (RF)sP[v * np + (l + n * (j + n * k))];
-      geta[v] += deta * (RF)sP[v * np + (i + n * (l + n * k))];
-      gzeta[v] += dzeta * (RF)sP[v * np + (i + n * (j + n * l))];
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) g[4 * d + v] = Ja[d] * gxi[v] + Ja[3 + d] * geta[v] + Ja[6 + d] * gzeta[v];
-  const int sides = node_sides<N>(i, j, k);
-  if (sides) {
-    const RF lhm = (RF)sB[B.L_HatMinus()], lhp = (RF)sB[B.L_HatPlus() + N];
-    const RF own[4] = {(RF)sP[t], (RF)sP[np + t], (RF)sP[2 * np + t], (RF)sP[3 * np + t]};
-#pragma unroll 1
-    for (int ls = 1; ls <= 6; ++ls) {
-      if (!(sides & (1 << (ls - 1)))) continue;
-      const int x = face_point<N>(ls, i, j, k);
-      const int dir = dir_of(ls);
-      const RF s = (RF)normal_sign(ls) * RF(0.5) * (minus_side(ls) ? lhm : lhp);
-      const RF nse[3] = {Ja[3 * dir] * s, Ja[3 * dir + 1] * s, Ja[3 * dir + 2] * s};
-      RF dq[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) dq[v] = (RF)sNb[v * NF + x] - own[v];
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) g[4 * d + v] += dq[v] * nse[d];
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 12; ++c) g[c] *= sj;
-}
-
-template <class RF>
-__device__ __forceinline__ void load_metrics(const DevMesh& m, size_t gid, RF* Ja, RF& sj) {
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    Ja[c] = (RF)m.Mf[3 * gid + c];
-    Ja[3 + c] = (RF)m.Mg[3 * gid + c];
-    Ja[6 + c] = (RF)m.Mh[3 * gid + c];
-  }
-  sj = (RF)m.sJ[gid];
-}
-
-// Metric terms of node t from the element's block of the packed array (rows of np: Ja^1 x,y,z,
-// Ja^2 x,y,z, Ja^3 x,y,z, sJ), so that consecutive lanes load consecutive doubles.
-template <int N, class RF>
-__device__ __forceinline__ void load_metrics(const double* Je, int t, RF* Ja, RF& sj) {
-  constexpr int np = D<N>::np;
-#pragma unroll
-  for (int r = 0; r < 9; ++r) Ja[r] = (RF)Je[r * np + t];
-  sj = (RF)Je[9 * np + t];
-}
 
 // ---------------------------------------------------------------------------------------------
 // Kernel A: viscous normal fluxes of the element's faces.
@@ -285,6 +143,8 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
     RF gn[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) gn[c] = FG[4 * (size_t)nb + c];
+    // (the same arithmetic as face_frame, written out: the register allocation of this kernel
+    // stays within 128 VGPRs this way)
     double se;
     RF nv[3];  // master normal
     if (m.ownFaceMetric) {
@@ -531,6 +391,17 @@ __global__ __launch_bounds__(256) void k_pack_metrics(DevMesh m, int np, size_t 
 }
 
 }  // namespace
+
+hipError_t launch_faces_of(const DevMesh& m, const double* U, double* FU, hipStream_t s) {
+  switch (m.N) {
+#define FXG_CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_faces_of<NN>), dim3(m.nElems), dim3(D<NN>::NT), 0, s, U, m, FU); break;
+    FXG_CASE(2) FXG_CASE(3) FXG_CASE(4) FXG_CASE(5) FXG_CASE(6) FXG_CASE(7)
+#undef FXG_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_pack_metrics(const DevMesh& m, double* Jsoa, hipStream_t s) {
   const int np = (m.N + 1) * (m.N + 1) * (m.N + 1);

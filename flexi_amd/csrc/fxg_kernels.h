@@ -91,6 +91,8 @@ struct FusedWork {
   void* FG;       // viscous normal face fluxes f_v.(n SurfElem), 4 per side point and slot
   double* Jsoa;   // metric terms per element as rows of (N+1)^3: f x,y,z, g x,y,z, h x,y,z, sJ
 };
+// Face states of U into a face array laid out like FusedWork::FU (Gauss-Lobatto).
+hipError_t launch_faces_of(const DevMesh& m, const double* U, double* FU, hipStream_t s);
 // Fills FusedWork::Jsoa from the FLEXI-layout metric arrays of `m`.
 hipError_t launch_pack_metrics(const DevMesh& m, double* Jsoa, hipStream_t s);
 bool fused_supported(int N, int nodeType, int split, long long nSides);
@@ -100,6 +102,20 @@ bool fused_supported(int N, int nodeType, int split, long long nSides);
 hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, int fp32,
                             const double* U, double* Ut, const RKStage& st, bool faces_fresh, int cur,
                             void* G, hipStream_t s, KernelTimer* timer);
+
+// Scratch of the discrete adjoint of the Gauss-Lobatto split-form right-hand side
+// (fxg_adjoint.hip); face arrays use the FusedWork::FU slot layout.
+struct AdjointWork {
+  double* FUa;  // face states of the linearisation state, 5 per side slot
+  double* XU;   // Riemann-flux adjoint w.r.t. the neighbour's face state, 5 per side slot
+  double* Xg;   // adjoint of the neighbour's viscous normal face flux, 4 per side slot
+  double* Xq;   // lifting adjoint w.r.t. the neighbour's face values (u,v,w,T), 4 per side slot
+};
+// out = (dUt/dU)^T lam at U (FLEXI layout, FP64, Gauss-Lobatto split form, N 2..7).
+hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
+                          const double* lam, double* out, hipStream_t s);
+hipError_t launch_axpy(size_t n, double a, const double* x, double* y, hipStream_t s);
+hipError_t launch_scale(size_t n, double a, double* y, hipStream_t s);
 
 hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U,
                                 double CFL, double DFL, hipStream_t s, KernelTimer* timer);

@@ -167,6 +167,29 @@ class DGSolver:
         _lib.check(_lib.lib().fxg_timestep_lserkw2_dev(self.ctx, float(t), float(dt), self.scheme.nStages, self._RKA,
                                                        self._RKb, self._RKc, U_ptr, R_ptr, stream or None))
 
+    # -- discrete adjoint (Gauss-Lobatto split form, FP64) -----------------------------------------
+    def rhs_vjp_dev(self, U_ptr: int, lam_ptr: int, out_ptr: int, t: float = 0.0, stream: int = 0) -> None:
+        """out = (dUt/dU)^T lam at U (device pointers, FLEXI layout)."""
+        _lib.check(_lib.lib().fxg_rhs_vjp_dev(self.ctx, float(t), U_ptr, lam_ptr, out_ptr, stream or None))
+
+    def _dt_array(self, dts):
+        dts = np.ascontiguousarray(np.asarray(dts, dtype=np.float64))
+        return dts, dts.ctypes.data_as(C.POINTER(C.c_double))
+
+    def forward_n_steps_dev(self, U_ptr: int, dts, stream: int = 0) -> None:
+        """len(dts) LSERK steps on the device state at U_ptr with the given time steps."""
+        arr, p = self._dt_array(dts)
+        _lib.check(_lib.lib().fxg_forward_n_steps(self.ctx, len(arr), p, self.scheme.nStages, self._RKA, self._RKb,
+                                                  self._RKc, U_ptr, stream or None))
+
+    def adjoint_n_steps_dev(self, U0_ptr: int, ubar_ptr: int, dts, checkpoint_every: int = 20,
+                            stream: int = 0) -> None:
+        """ubar <- (dU(T)/dU(0))^T ubar for the steps of forward_n_steps_dev from U0."""
+        arr, p = self._dt_array(dts)
+        _lib.check(_lib.lib().fxg_adjoint_n_steps(self.ctx, len(arr), p, self.scheme.nStages, self._RKA, self._RKb,
+                                                  self._RKc, U0_ptr, ubar_ptr, int(checkpoint_every),
+                                                  stream or None))
+
     def kernel_pipeline(self) -> str:
         """'fused' or 'classic' (which stage kernels this context launches)."""
         return _lib.lib().fxg_pipeline(self.ctx).decode()

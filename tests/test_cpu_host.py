@@ -158,3 +158,13 @@ def test_fortran_binding_compiles_links_and_reports_errors(tmp_path):
     sizes = dict(line.split("=") for line in out.splitlines() if line.startswith("sizeof"))
     assert int(sizes["sizeof(fxg_params)"]) == ctypes.sizeof(_lib.FxgParams)
     assert int(sizes["sizeof(fxg_mesh)"]) == ctypes.sizeof(_lib.FxgMesh)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 10])
+def test_host_quadrature_matches_oracle(N):
+    from flexi_amd.basis import nodes_weights
+    for node in (orc.NODE_G, orc.NODE_GL):
+        x, w = nodes_weights(N, node)
+        xo, wo = orc.nodes_weights(N, node)[:2]
+        np.testing.assert_allclose(x, xo, rtol=0, atol=1e-14)
+        np.testing.assert_allclose(w, wo, rtol=1e-13, atol=1e-15)
