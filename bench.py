@@ -6,8 +6,13 @@ Taylor-Green vortex (Ma 0.1, Re 1600) on a periodic 32^3 hexahedral mesh, N = 7 
 nodes (16,777,216 DOF), split form with the Pirozzoli flux, Roe Riemann solver, BR1 lifting.
 DOF = one volume node (FLEXI's convention); value = DOF * stages * steps / wall time.
 
+Also measured (keys of the same JSON line): the FP32-bulk rate and its error against FP64
+("fp32_bulk"), the adjoint/forward time ratio of the kinetic-energy gradient on TGV 16^3 N=5
+over 200 steps ("adjoint"), and a curvilinear 24^3 N=5 case ("curvilinear").
+
 Prints one JSON line. Options: --gpus (only 1 exists on this platform), --steps, --warmup,
---elems (per direction), --no-cpu (skip the CPU baseline), --no-fp32.
+--elems (per direction), --no-cpu (skip the CPU baseline), --no-fp32, --no-adjoint,
+--no-curvilinear.
 """
 from __future__ import annotations
 
@@ -37,6 +42,10 @@ ALGO_BYTES_PER_DOF = {"prolong_u": 60.0, "lift": 372.0, "flux": 147.0, "volint":
                       "faces": 53.125, "visc_faces": 140.25, "dg_fused": 316.0}
 # Per DOF-stage byte model of the stage (U, Ut/R, metrics and gradient traffic of one stage)
 B_STAGE = 432.0
+# HBM bytes per DOF of one launch of the fused kernels at this workload (FP64), from rocprofv3
+# FETCH_SIZE (x2.0, plain-copy calibration) + WRITE_SIZE runs of this benchmark; the summary and
+# the commands are in profiles/fused_stage_tgv32_n7_fp64.md
+MEASURED_BYTES_PER_DOF = {"dg_fused": 380.8, "visc_faces": 174.3, "faces": 61.3}
 
 
 def make_solver(nel, precision):
@@ -82,6 +91,107 @@ def s_tgv(s):
     return testcase.tgv(xgp)
 
 
+def fp32_accuracy():
+    """FP32-bulk against FP64 on TGV 8^3, N=3 Gauss, weak form, LF, density perturbed by 1e-3
+    (seed 1), 1000 steps of the FP64 run's first time step: relative L2 error of U and of the
+    kinetic-energy dissipation rate -dEkin/dt (from the right-hand side at the final state)."""
+    from flexi_amd import testcase
+    from flexi_amd.basis import nodes_weights
+    from flexi_amd.mesh import periodic_box
+    from flexi_amd.solver import DGSolver
+    mesh = periodic_box(8, 8, 8)
+    res, dt = {}, None
+    for prec in ("fp64", "fp32-bulk"):
+        s = DGSolver(mesh, 3, node_type="gauss", split_dg=None, riemann="lf", precision=prec)
+        m = s.metrics()
+        s.set_state(testcase.tgv_density_perturbed(m["Elem_xGP"], amplitude=1e-3, seed=1))
+        if dt is None:
+            dt = s.CalcTimeStep()[0]
+        s.run(1000, fixed_dt=dt)
+        U = s.get_state()
+        s.DGTimeDerivative_weakForm(1000 * dt)
+        res[prec] = (U, s.get_ut(), m["sJ"])
+        s.close()
+    (U64, Ut64, sJ), (U32, Ut32, _) = res["fp64"], res["fp32-bulk"]
+    w = nodes_weights(3, 1)[1]
+    W = (w[:, None, None] * w[None, :, None] * w[None, None, :])[..., None] / sJ
+
+    def dek(U, Ut):
+        u = U[1:4] / U[0]
+        return -np.sum(W * (np.sum(u * Ut[1:4], axis=0) - 0.5 * np.sum(u * u, axis=0) * Ut[0]))
+    d64, d32 = dek(U64, Ut64), dek(U32, Ut32)
+    return {"rel_l2_U_vs_fp64": float(np.linalg.norm(U32 - U64) / np.linalg.norm(U64)),
+            "rel_err_dEkin_dt_vs_fp64": float(abs(d32 - d64) / abs(d64)),
+            "accuracy_case": "TGV 8^3, N=3 Gauss, weak, LF, rho perturbed 1e-3, 1000 steps"}
+
+
+def adjoint_ratio():
+    """Gradient of the kinetic energy at T w.r.t. U(0): TGV 16^3, N=5 Gauss-Lobatto split PI, Roe,
+    200 steps (dt frozen from the primal run), checkpoints every 20 steps. Ratio = time of the
+    adjoint call (its own forward sweep, the recomputation and the reverse sweep) / time of the
+    200 forward steps."""
+    import torch
+    from flexi_amd import testcase
+    from flexi_amd.adjoint import primal_dts, to_tensor
+    from flexi_amd.mesh import periodic_box
+    from flexi_amd.solver import DGSolver
+    mesh = periodic_box(16, 16, 16)
+    s = DGSolver(mesh, 5, node_type="gauss-lobatto", split_dg="pi", riemann="roe")
+    U0 = testcase.tgv(s.metrics()["Elem_xGP"])
+    dts = primal_dts(s, U0, 200)
+    U0d = to_tensor(U0)
+    U = U0d.clone()
+    ubar = torch.ones_like(U0d)
+    torch.cuda.synchronize()
+    s.forward_n_steps_dev(U.data_ptr(), dts[:5])  # warm-up
+    s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts[:5], 20)
+    s.sync()
+    U = U0d.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.forward_n_steps_dev(U.data_ptr(), dts)
+    s.sync()
+    t_fwd = time.perf_counter() - t0
+    ubar = torch.ones_like(U0d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts, 20)
+    t_adj = time.perf_counter() - t0
+    state = U0.nbytes
+    s.close()
+    return {"ratio": t_adj / t_fwd, "forward_s": t_fwd, "adjoint_s": t_adj,
+            "checkpoint_bytes": (10 + 20 * 5 + 3) * state,
+            "case": "TGV 16^3, N=5 GL split PI Roe, 200 steps, checkpoints every 20 steps"}
+
+
+def curvilinear():
+    """24^3 periodic box mapped by x_i + 0.1 sin x sin y sin z, N=5 Gauss-Lobatto split PI, Roe,
+    metrics on the GPU: freestream preservation after 100 steps, then a Gaussian density pulse
+    (amplitude 0.1, width 0.5) on that freestream, 50 timed steps."""
+    from flexi_amd import testcase
+    from flexi_amd.mesh import periodic_box, sine_deform
+    from flexi_amd.solver import DGSolver
+    N = 5
+    mesh = periodic_box(24, 24, 24, NGeo=N, deform=sine_deform(0.1))
+    s = DGSolver(mesh, N, node_type="gauss-lobatto", split_dg="pi", riemann="roe")
+    n = N + 1
+    U0 = testcase.freestream((n, n, n, mesh.nElems), vel=(0.3, 0.2, 0.1), p=1.0 / 1.4)
+    s.set_state(U0)
+    s.run(100)
+    fs_err = float(np.max(np.abs(s.get_state() - U0)))
+    s.set_state(testcase.gaussian_pulse(s.metrics()["Elem_xGP"]))
+    s.run(5)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(50)
+    s.sync()
+    wall = time.perf_counter() - t0
+    dof = mesh.nElems * n ** 3
+    s.close()
+    return {"freestream_max_abs_err_100_steps": fs_err, "pulse_value": dof * 5 * 50 / wall,
+            "unit": "DOF-stage/s", "dof": dof, "ms_per_step": 1e3 * wall / 50}
+
+
 def cpu_baseline(nel):
     """The CPU oracle (a C++/OpenMP restatement of the reference) on one LSERK step of the same
     workload, on the host cores of this machine."""
@@ -117,6 +227,8 @@ def main():
     ap.add_argument("--elems", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-curvilinear", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (calibrates HBM counters)")
     args = ap.parse_args()
@@ -168,7 +280,9 @@ def main():
                    "dof": r64["dof"], "stages_per_step": stages, "parallelism": "single GPU",
                    "pipeline": r64["pipeline"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (MEASURED_BYTES_PER_DOF[dom] * r64["dof"]
+                                 if dom in MEASURED_BYTES_PER_DOF and args.elems == 32 else None),
                      "algorithmic_bytes_per_dof": ALGO_BYTES_PER_DOF[dom], "avg_launch_ms": avg_s * 1e3},
         "roofline_stage": {"bytes_per_dof_stage": B_STAGE, "stage_ms": stage_s * 1e3, "achieved": stage_achieved,
                            "unit": "GB/s", "frac": stage_achieved / HBM_PEAK_GBS,
@@ -180,8 +294,17 @@ def main():
         print("[bench] fp32-bulk", file=sys.stderr, flush=True)
         r32 = run_gpu(args.elems, "fp32-bulk", args.steps, args.warmup, profile=False)
         out["fp32_bulk"] = {"value": r32["dof"] * stages * args.steps / r32["wall"], "unit": "DOF-stage/s",
-                            "ms_per_step": 1e3 * r32["wall"] / args.steps}
-    out["adjoint_forward_ratio"] = None
+                            "ms_per_step": 1e3 * r32["wall"] / args.steps, **fp32_accuracy()}
+    if not args.no_adjoint:
+        print("[bench] adjoint (TGV 16^3 N=5, 200 steps)", file=sys.stderr, flush=True)
+        adj = adjoint_ratio()
+        out["adjoint_forward_ratio"] = adj["ratio"]
+        out["adjoint"] = adj
+    else:
+        out["adjoint_forward_ratio"] = None
+    if not args.no_curvilinear:
+        print("[bench] curvilinear 24^3 N=5", file=sys.stderr, flush=True)
+        out["curvilinear"] = curvilinear()
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, one step)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.elems)
