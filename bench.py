@@ -127,40 +127,45 @@ def fp32_accuracy():
 
 def adjoint_ratio():
     """Gradient of the kinetic energy at T w.r.t. U(0): TGV 16^3, N=5 Gauss-Lobatto split PI, Roe,
-    200 steps (dt frozen from the primal run), checkpoints every 20 steps. Ratio = time of the
-    adjoint call (its own forward sweep, the recomputation and the reverse sweep) / time of the
-    200 forward steps."""
+    200 steps (dt frozen from the primal run), checkpoints every 20 steps. `ratio` = time of the
+    adjoint (backward) call, which recomputes the stage states of each 20-step segment from the
+    forward's checkpoints and runs the reverse stages, / time of the 200 primal forward steps."""
     import torch
     from flexi_amd import testcase
     from flexi_amd.adjoint import primal_dts, to_tensor
     from flexi_amd.mesh import periodic_box
     from flexi_amd.solver import DGSolver
+    every = 20
     mesh = periodic_box(16, 16, 16)
     s = DGSolver(mesh, 5, node_type="gauss-lobatto", split_dg="pi", riemann="roe")
     U0 = testcase.tgv(s.metrics()["Elem_xGP"])
     dts = primal_dts(s, U0, 200)
     U0d = to_tensor(U0)
+    ckpt = torch.empty((len(dts) // every,) + tuple(U0d.shape), dtype=U0d.dtype, device=U0d.device)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        s.sync()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
     U = U0d.clone()
+    timed(lambda: s.forward_n_steps_dev(U.data_ptr(), dts[:5]))  # warm-up
     ubar = torch.ones_like(U0d)
-    torch.cuda.synchronize()
-    s.forward_n_steps_dev(U.data_ptr(), dts[:5])  # warm-up
-    s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts[:5], 20)
-    s.sync()
+    timed(lambda: s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts[:5], every))
     U = U0d.clone()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.forward_n_steps_dev(U.data_ptr(), dts)
-    s.sync()
-    t_fwd = time.perf_counter() - t0
+    t_fwd = timed(lambda: s.forward_n_steps_dev(U.data_ptr(), dts))
+    U = U0d.clone()
+    t_fwd_ck = timed(lambda: s.forward_n_steps_dev(U.data_ptr(), dts, ckpt.data_ptr(), every))
     ubar = torch.ones_like(U0d)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts, 20)
-    t_adj = time.perf_counter() - t0
+    t_bwd = timed(lambda: s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts, every, ckpt.data_ptr()))
     state = U0.nbytes
     s.close()
-    return {"ratio": t_adj / t_fwd, "forward_s": t_fwd, "adjoint_s": t_adj,
-            "checkpoint_bytes": (10 + 20 * 5 + 3) * state,
+    return {"ratio": t_bwd / t_fwd, "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "adjoint_s": t_bwd,
+            "gradient_total_ratio": (t_fwd_ck + t_bwd) / t_fwd,
+            "memory_bytes": {"checkpoints": ckpt.numel() * 8, "segment_stage_states": every * 5 * state,
+                             "work": 3 * state},
             "case": "TGV 16^3, N=5 GL split PI Roe, 200 steps, checkpoints every 20 steps"}
 
 

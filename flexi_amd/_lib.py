@@ -63,8 +63,8 @@ def lib() -> C.CDLL:
     L.fxg_dg_time_derivative_dev.argtypes = [vp, C.c_double, vp, vp, vp]
     L.fxg_timestep_lserkw2_dev.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, dp, dp, vp, vp, vp]
     L.fxg_rhs_vjp_dev.argtypes = [vp, C.c_double, vp, vp, vp, vp]
-    L.fxg_forward_n_steps.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, vp, vp]
-    L.fxg_adjoint_n_steps.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, vp, vp, C.c_int, vp]
+    L.fxg_forward_n_steps.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, vp, vp, C.c_int, vp]
+    L.fxg_adjoint_n_steps.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, vp, vp, vp, C.c_int, vp]
     L.fxg_pipeline.argtypes = [vp]
     L.fxg_pipeline.restype = C.c_char_p
     L.fxg_sync.argtypes = [vp]

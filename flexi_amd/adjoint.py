@@ -58,20 +58,24 @@ class LSERKSteps(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, U0: torch.Tensor, solver: DGSolver, dts, checkpoint_every: int):
+        dts = np.asarray(dts, dtype=np.float64)
+        every = int(checkpoint_every)
         U = U0.detach().clone().contiguous()
+        # checkpoints of the state every `every` steps, kept for the backward pass
+        ckpt = torch.empty(((len(dts) + every - 1) // every,) + tuple(U.shape), dtype=U.dtype, device=U.device)
         torch.cuda.synchronize()  # the library runs on its own stream
-        solver.forward_n_steps_dev(U.data_ptr(), dts)
+        solver.forward_n_steps_dev(U.data_ptr(), dts, ckpt.data_ptr(), every)
         solver.sync()
-        ctx.save_for_backward(U0.detach().contiguous())
-        ctx.solver, ctx.dts, ctx.every = solver, np.asarray(dts, dtype=np.float64), int(checkpoint_every)
+        ctx.save_for_backward(U0.detach().contiguous(), ckpt)
+        ctx.solver, ctx.dts, ctx.every = solver, dts, every
         return U
 
     @staticmethod
     def backward(ctx, gU: torch.Tensor):
-        (U0,) = ctx.saved_tensors
+        U0, ckpt = ctx.saved_tensors
         ubar = gU.detach().clone().contiguous()
         torch.cuda.synchronize()
-        ctx.solver.adjoint_n_steps_dev(U0.data_ptr(), ubar.data_ptr(), ctx.dts, ctx.every)
+        ctx.solver.adjoint_n_steps_dev(U0.data_ptr(), ubar.data_ptr(), ctx.dts, ctx.every, ckpt.data_ptr())
         return ubar, None, None, None
 
 

@@ -23,6 +23,12 @@
 #include "fxg_dual.h"
 #include "fxg_elem.h"
 
+// FXG_ADJ_DUAL=1 takes the Riemann-flux derivatives from dual-number evaluations instead of the
+// hand-written reverse (a cross-check of the reverse; slower).
+#ifndef FXG_ADJ_DUAL
+#define FXG_ADJ_DUAL 0
+#endif
+
 namespace fxg {
 namespace {
 
@@ -56,10 +62,245 @@ __device__ __forceinline__ RotState<Dual> rot_state(const Dual* c, double kappaM
   return rotate<Dual>(c[0], c[4], p, u, v, w, nv, t1, t2);
 }
 
+// Reverse of riemann_rot (split form) at one side point: lF is the adjoint of F in the side
+// frame; the adjoints of the two rotated states' (rho, E, p, vn, vt1, vt2) are returned in aL, aR.
+// Line by line the transpose of the forward arithmetic in fxg_physics.h (riemann.f90:558-855,
+// splitflux.f90:672-700); srho = 1/rho is folded into rho.
+struct RotAdj {
+  double rho, E, p, vn, vt1, vt2;
+};
+
+template <int SOLVER>
+__device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<double>& L, const RotState<double>& R,
+                                                const double* lF, RotAdj& aL, RotAdj& aR) {
+  aL = RotAdj{0, 0, 0, 0, 0, 0};
+  aR = RotAdj{0, 0, 0, 0, 0, 0};
+  // central part, Pirozzoli surface flux: F += Fc
+  {
+    const double HL = (L.E + L.p) / L.rho, HR = (R.E + R.p) / R.rho;
+    const double rs = L.rho + R.rho, us = L.vn + R.vn, sv1 = L.vt1 + R.vt1, sv2 = L.vt2 + R.vt2, Hs = HL + HR;
+    const double lrs = 0.25 * us * lF[0] + 0.125 * us * us * lF[1] + 0.125 * us * sv1 * lF[2] +
+                       0.125 * us * sv2 * lF[3] + 0.125 * Hs * us * lF[4];
+    const double lus = 0.25 * rs * lF[0] + 0.25 * rs * us * lF[1] + 0.125 * rs * sv1 * lF[2] +
+                       0.125 * rs * sv2 * lF[3] + 0.125 * rs * Hs * lF[4];
+    const double lsv1 = 0.125 * rs * us * lF[2], lsv2 = 0.125 * rs * us * lF[3], lHs = 0.125 * rs * us * lF[4];
+    aL.p += 0.5 * lF[1]; aR.p += 0.5 * lF[1];
+    aL.rho += lrs - HL * lHs / L.rho; aR.rho += lrs - HR * lHs / R.rho;
+    aL.vn += lus; aR.vn += lus;
+    aL.vt1 += lsv1; aR.vt1 += lsv1;
+    aL.vt2 += lsv2; aR.vt2 += lsv2;
+    aL.E += lHs / L.rho; aR.E += lHs / R.rho;
+    aL.p += lHs / L.rho; aR.p += lHs / R.rho;
+  }
+  const double UL[5] = {L.rho, L.rho * L.vn, L.rho * L.vt1, L.rho * L.vt2, L.E};
+  const double UR[5] = {R.rho, R.rho * R.vn, R.rho * R.vt1, R.rho * R.vt2, R.E};
+  double ld[5] = {0, 0, 0, 0, 0};  // adjoint of UR - UL
+  // d(sqrt(kappa p / rho)): c = sqrt(kappa p srho)
+  auto sound_adj = [&](const RotState<double>& S, double c, double lc, RotAdj& a) {
+    const double f = 0.5 * g.kappa / c;
+    a.p += f * S.srho * lc;
+    a.rho -= f * S.p * S.srho * S.srho * lc;
+  };
+  if (SOLVER == 1) {  // Lax-Friedrichs: F -= 0.5*lmax*(UR - UL)
+    const double cL = sqrt(g.kappa * L.p * L.srho), cR = sqrt(g.kappa * R.p * R.srho);
+    const double vmax = fmax(fabs(R.vn), fabs(L.vn)), cmax = fmax(cL, cR);
+    const double lmax = vmax + cmax;
+    double llmax = 0.0;
+    for (int k = 0; k < 5; ++k) {
+      ld[k] = -0.5 * lmax * lF[k];
+      llmax -= 0.5 * (UR[k] - UL[k]) * lF[k];
+    }
+    if (fabs(R.vn) >= fabs(L.vn)) aR.vn += (R.vn < 0.0 ? -llmax : llmax);
+    else aL.vn += (L.vn < 0.0 ? -llmax : llmax);
+    if (cL >= cR) sound_adj(L, cL, llmax, aL);
+    else sound_adj(R, cR, llmax, aR);
+  } else {
+    // Roe averages
+    const double HLr = (L.E + L.p) * L.srho, HRr = (R.E + R.p) * R.srho;
+    const double sqL = sqrt(L.rho), sqR = sqrt(R.rho);
+    const double ss = 1.0 / (sqL + sqR);
+    const double ru = (sqR * R.vn + sqL * L.vn) * ss;
+    const double rv = (sqR * R.vt1 + sqL * L.vt1) * ss;
+    const double rw = (sqR * R.vt2 + sqL * L.vt2) * ss;
+    const double absVel = ru * ru + rv * rv + rw * rw;
+    const double H = (sqR * HRr + sqL * HLr) * ss;
+    const double c = sqrt(g.kappaM1 * (H - 0.5 * absVel));
+    double al[5], a[5];
+    double d[5];
+    for (int k = 0; k < 5; ++k) d[k] = UR[k] - UL[k];
+    // entropy-fix quantities
+    double cL = 0, cR = 0, rd = 0, tmp = 0, du = 0, dp = 0;
+    if (SOLVER == 3) {
+      const double d5 = d[4] - (d[2] - rv * d[0]) * rv - (d[3] - rw * d[0]) * rw;
+      al[2] = d[2] - rv * d[0];
+      al[3] = d[3] - rw * d[0];
+      al[1] = g.kappaM1 / (c * c) * (d[0] * (H - ru * ru) - d5 + ru * d[1]);
+      al[0] = 0.5 / c * (d[0] * (ru + c) - d[1] - c * al[1]);
+      al[4] = d[0] - al[0] - al[1];
+      a[0] = fabs(ru - c); a[1] = a[2] = a[3] = fabs(ru); a[4] = fabs(ru + c);
+    } else {
+      cL = sqrt(g.kappa * L.p * L.srho); cR = sqrt(g.kappa * R.p * R.srho);
+      rd = sqrt(L.rho * R.rho);
+      const double d1 = R.rho - L.rho, dv = R.vt1 - L.vt1, dw = R.vt2 - L.vt2;
+      du = R.vn - L.vn; dp = R.p - L.p;
+      tmp = 0.5 / (c * c);
+      al[0] = tmp * (dp - rd * c * du);
+      al[1] = d1 - dp * 2.0 * tmp;
+      al[2] = rd * dv;
+      al[3] = rd * dw;
+      al[4] = tmp * (dp + rd * c * du);
+      const double ar[5] = {ru - c, ru, ru, ru, ru + c};
+      const double aLk[5] = {L.vn - cL, L.vn, L.vn, L.vn, L.vn + cL};
+      const double aRk[5] = {R.vn - cR, R.vn, R.vn, R.vn, R.vn + cR};
+      for (int k = 0; k < 5; ++k) {
+        const double da = fmax(fmax(0.0, ar[k] - aLk[k]), aRk[k] - ar[k]);
+        a[k] = (fabs(ar[k]) < da) ? 0.5 * (ar[k] * ar[k] / da + da) : fabs(ar[k]);
+      }
+    }
+    const double w[5] = {al[0] * a[0], al[1] * a[1], al[2] * a[2], al[3] * a[3], al[4] * a[4]};
+    // F -= 0.5*D with D the dissipation sum_k w_k r_k
+    double lD[5];
+    for (int k = 0; k < 5; ++k) lD[k] = -0.5 * lF[k];
+    double lw[5];
+    lw[0] = lD[0] + (ru - c) * lD[1] + rv * lD[2] + rw * lD[3] + (H - ru * c) * lD[4];
+    lw[1] = lD[0] + ru * lD[1] + rv * lD[2] + rw * lD[3] + 0.5 * absVel * lD[4];
+    lw[2] = lD[2] + rv * lD[4];
+    lw[3] = lD[3] + rw * lD[4];
+    lw[4] = lD[0] + (ru + c) * lD[1] + rv * lD[2] + rw * lD[3] + (H + ru * c) * lD[4];
+    const double wsum = w[0] + w[1] + w[4];
+    double lru = lD[1] * wsum + lD[4] * (w[4] - w[0]) * c;
+    double lc = lD[1] * (w[4] - w[0]) + lD[4] * (w[4] - w[0]) * ru;
+    double lrv = lD[2] * wsum + lD[4] * w[2];
+    double lrw = lD[3] * wsum + lD[4] * w[3];
+    double lH = lD[4] * (w[0] + w[4]);
+    double labs = 0.5 * w[1] * lD[4];
+    double lal[5], la[5];
+    for (int k = 0; k < 5; ++k) {
+      lal[k] = a[k] * lw[k];
+      la[k] = al[k] * lw[k];
+    }
+    if (SOLVER == 3) {
+      // al4 = d0 - al0 - al1
+      ld[0] += lal[4]; lal[0] -= lal[4]; lal[1] -= lal[4];
+      // al0 = 0.5/c*Y, Y = d0(ru+c) - d1 - c al1
+      const double Y = d[0] * (ru + c) - d[1] - c * al[1];
+      const double lY = 0.5 * lal[0] / c;
+      lc -= al[0] / c * lal[0];
+      ld[0] += (ru + c) * lY; lru += d[0] * lY; lc += (d[0] - al[1]) * lY; ld[1] -= lY; lal[1] -= c * lY;
+      (void)Y;
+      // al1 = kappaM1/c^2*X, X = d0(H - ru^2) - d5 + ru d1
+      const double lX = g.kappaM1 * lal[1] / (c * c);
+      lc -= 2.0 * al[1] / c * lal[1];
+      double ld5 = -lX;
+      ld[0] += (H - ru * ru) * lX; lH += d[0] * lX; lru += (d[1] - 2.0 * ru * d[0]) * lX; ld[1] += ru * lX;
+      // al2 = d2 - rv d0, al3 = d3 - rw d0
+      ld[2] += lal[2]; lrv -= d[0] * lal[2]; ld[0] -= rv * lal[2];
+      ld[3] += lal[3]; lrw -= d[0] * lal[3]; ld[0] -= rw * lal[3];
+      // d5 = d4 - e2 rv - e3 rw, e2 = d2 - rv d0, e3 = d3 - rw d0
+      const double e2 = d[2] - rv * d[0], e3 = d[3] - rw * d[0];
+      ld[4] += ld5;
+      const double le2 = -rv * ld5, le3 = -rw * ld5;
+      lrv -= e2 * ld5; lrw -= e3 * ld5;
+      ld[2] += le2; lrv -= d[0] * le2; ld[0] -= rv * le2;
+      ld[3] += le3; lrw -= d[0] * le3; ld[0] -= rw * le3;
+      // a = |ru -/+ c|, |ru|
+      const double s0 = (ru - c) < 0.0 ? -1.0 : 1.0, s1 = ru < 0.0 ? -1.0 : 1.0, s4 = (ru + c) < 0.0 ? -1.0 : 1.0;
+      lru += s0 * la[0] + s1 * (la[1] + la[2] + la[3]) + s4 * la[4];
+      lc += -s0 * la[0] + s4 * la[4];
+    } else {
+      // entropy-fixed |a_k|
+      const double ar[5] = {ru - c, ru, ru, ru, ru + c};
+      const double aLk[5] = {L.vn - cL, L.vn, L.vn, L.vn, L.vn + cL};
+      const double aRk[5] = {R.vn - cR, R.vn, R.vn, R.vn, R.vn + cR};
+      double lar[5] = {0, 0, 0, 0, 0}, laL[5] = {0, 0, 0, 0, 0}, laR[5] = {0, 0, 0, 0, 0};
+      for (int k = 0; k < 5; ++k) {
+        const double x = ar[k] - aLk[k], y = aRk[k] - ar[k];
+        const double m1 = fmax(0.0, x);
+        const double da = fmax(m1, y);
+        double lda = 0.0;
+        if (fabs(ar[k]) < da) {
+          lar[k] += ar[k] / da * la[k];
+          lda = 0.5 * (1.0 - ar[k] * ar[k] / (da * da)) * la[k];
+        } else {
+          lar[k] += (ar[k] < 0.0 ? -la[k] : la[k]);
+        }
+        if (m1 >= y) {
+          if (x > 0.0) { lar[k] += lda; laL[k] -= lda; }
+        } else {
+          laR[k] += lda; lar[k] -= lda;
+        }
+      }
+      lru += lar[0] + lar[1] + lar[2] + lar[3] + lar[4];
+      lc += -lar[0] + lar[4];
+      aL.vn += laL[0] + laL[1] + laL[2] + laL[3] + laL[4];
+      aR.vn += laR[0] + laR[1] + laR[2] + laR[3] + laR[4];
+      double lcL = -laL[0] + laL[4], lcR = -laR[0] + laR[4];
+      // wave strengths from primitive jumps
+      const double dv = R.vt1 - L.vt1, dw = R.vt2 - L.vt2;
+      double ltmp = (dp - rd * c * du) * lal[0] + (dp + rd * c * du) * lal[4] - 2.0 * dp * lal[1];
+      const double ldp = tmp * (lal[0] + lal[4]) - 2.0 * tmp * lal[1];
+      const double lrd = -tmp * c * du * lal[0] + tmp * c * du * lal[4] + dv * lal[2] + dw * lal[3];
+      lc += -tmp * rd * du * lal[0] + tmp * rd * du * lal[4];
+      const double ldu = -tmp * rd * c * lal[0] + tmp * rd * c * lal[4];
+      const double ld1 = lal[1];
+      const double ldv = rd * lal[2], ldw = rd * lal[3];
+      lc -= ltmp / (c * c * c);  // tmp = 0.5/c^2
+      aL.rho += R.rho / (2.0 * rd) * lrd; aR.rho += L.rho / (2.0 * rd) * lrd;
+      aR.rho += ld1; aL.rho -= ld1;
+      aR.vn += ldu; aL.vn -= ldu;
+      aR.vt1 += ldv; aL.vt1 -= ldv;
+      aR.vt2 += ldw; aL.vt2 -= ldw;
+      aR.p += ldp; aL.p -= ldp;
+      sound_adj(L, cL, lcL, aL);
+      sound_adj(R, cR, lcR, aR);
+    }
+    // c = sqrt(kappaM1 (H - absVel/2))
+    lH += 0.5 * g.kappaM1 / c * lc;
+    labs -= 0.25 * g.kappaM1 / c * lc;
+    lru += 2.0 * ru * labs; lrv += 2.0 * rv * labs; lrw += 2.0 * rw * labs;
+    // H, ru, rv, rw: Roe averages with weights sqL*ss, sqR*ss
+    double lsqL = HLr * ss * lH + L.vn * ss * lru + L.vt1 * ss * lrv + L.vt2 * ss * lrw;
+    double lsqR = HRr * ss * lH + R.vn * ss * lru + R.vt1 * ss * lrv + R.vt2 * ss * lrw;
+    const double lss = (sqR * HRr + sqL * HLr) * lH + (sqR * R.vn + sqL * L.vn) * lru +
+                       (sqR * R.vt1 + sqL * L.vt1) * lrv + (sqR * R.vt2 + sqL * L.vt2) * lrw;
+    lsqL -= ss * ss * lss; lsqR -= ss * ss * lss;
+    aL.vn += sqL * ss * lru; aR.vn += sqR * ss * lru;
+    aL.vt1 += sqL * ss * lrv; aR.vt1 += sqR * ss * lrv;
+    aL.vt2 += sqL * ss * lrw; aR.vt2 += sqR * ss * lrw;
+    const double lHL = sqL * ss * lH, lHR = sqR * ss * lH;
+    aL.E += L.srho * lHL; aL.p += L.srho * lHL; aL.rho -= (L.E + L.p) * L.srho * L.srho * lHL;
+    aR.E += R.srho * lHR; aR.p += R.srho * lHR; aR.rho -= (R.E + R.p) * R.srho * R.srho * lHR;
+    aL.rho += lsqL / (2.0 * sqL); aR.rho += lsqR / (2.0 * sqR);
+  }
+  // U = (rho, rho vn, rho vt1, rho vt2, E) of both sides through d = UR - UL
+  aR.rho += ld[0] + R.vn * ld[1] + R.vt1 * ld[2] + R.vt2 * ld[3];
+  aR.vn += R.rho * ld[1]; aR.vt1 += R.rho * ld[2]; aR.vt2 += R.rho * ld[3]; aR.E += ld[4];
+  aL.rho -= ld[0] + L.vn * ld[1] + L.vt1 * ld[2] + L.vt2 * ld[3];
+  aL.vn -= L.rho * ld[1]; aL.vt1 -= L.rho * ld[2]; aL.vt2 -= L.rho * ld[3]; aL.E -= ld[4];
+}
+
+// Adjoint of a rotated state (rho, E, p, vn, vt1, vt2) to the conservative state it was built
+// from (cons_to_prim + rotation by n, t1, t2).
+__device__ __forceinline__ void rot_state_adjoint(const double* c, const Gas& g, const double* nv, const double* t1,
+                                                  const double* t2, const RotAdj& a, double* lU) {
+  const double sr = 1.0 / c[0];
+  const double u = c[1] * sr, v = c[2] * sr, w = c[3] * sr;
+  const double lu = a.vn * nv[0] + a.vt1 * t1[0] + a.vt2 * t2[0];
+  const double lv = a.vn * nv[1] + a.vt1 * t1[1] + a.vt2 * t2[1];
+  const double lw = a.vn * nv[2] + a.vt1 * t1[2] + a.vt2 * t2[2];
+  // p = kappaM1 (E - (m.u)/2)
+  lU[4] = a.E + g.kappaM1 * a.p;
+  lU[1] = lu * sr - g.kappaM1 * u * a.p;
+  lU[2] = lv * sr - g.kappaM1 * v * a.p;
+  lU[3] = lw * sr - g.kappaM1 * w * a.p;
+  lU[0] = a.rho + g.kappaM1 * 0.5 * (u * u + v * v + w * w) * a.p - (u * lu + v * lv + w * lw) * sr;
+}
+
 // ---------------------------------------------------------------------------------------------
 template <int N, int SOLVER>
 __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict__ U, DevMesh m,
                                                       const double* __restrict__ FUa, const double* __restrict__ lam,
+                                                      const double* __restrict__ ub, double cub,
                                                       Gas gas, double* __restrict__ out, double* __restrict__ XU,
                                                       double* __restrict__ Xg) {
   constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
@@ -80,7 +321,7 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict
     sA[3 * np + t] = q.u; sA[4 * np + t] = q.v; sA[5 * np + t] = q.w;
     const double sj = m.sJ[gid];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) sMu[v * np + t] = -sj * lam[5 * gid + v];
+    for (int v = 0; v < 5; ++v) sMu[v * np + t] = -sj * (lam[5 * gid + v] + (ub ? cub * ub[5 * gid + v] : 0.0));
   }
   for (int x = t; x < NF; x += NT) sIdx[x] = face_slot<N>(m, e, x);
   __syncthreads();
@@ -103,6 +344,8 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict
     for (int v = 0; v < 5; ++v) lF[v] = ms * se * lh * sMu[v * np + id];
     const double* Uo = U + 5 * ((size_t)np * e + id);
     const double* Un = FUa + 5 * (size_t)nb;
+#if FXG_ADJ_DUAL
+    // directional derivatives in dual arithmetic (10 evaluations): the reference for the reverse
     const Dual nd[3] = {nv[0], nv[1], nv[2]}, t1d[3] = {t1[0], t1[1], t1[2]}, t2d[3] = {t2[0], t2[1], t2[2]};
     const GasT<Dual> gd(gas);
 #pragma unroll 1
@@ -124,6 +367,29 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict
       if (k < 5) sLf[k * NF + x] = s;
       else XU[5 * (size_t)own + (k - 5)] = s;
     }
+#else
+    // flux adjoint in the side frame, then the reverse of the Riemann solver and of the rotation
+    double lFr[5];
+    lFr[0] = lF[0];
+    lFr[1] = nv[0] * lF[1] + nv[1] * lF[2] + nv[2] * lF[3];
+    lFr[2] = t1[0] * lF[1] + t1[1] * lF[2] + t1[2] * lF[3];
+    lFr[3] = t2[0] * lF[1] + t2[1] * lF[2] + t2[2] * lF[3];
+    lFr[4] = lF[4];
+    const Prim po = cons_to_prim(Uo, gas), pn = cons_to_prim(Un, gas);
+    const RotState<double> So = rotate<double>(po.rho, Uo[4], po.p, po.u, po.v, po.w, nv, t1, t2);
+    const RotState<double> Sn = rotate<double>(pn.rho, Un[4], pn.p, pn.u, pn.v, pn.w, nv, t1, t2);
+    RotAdj ao, an;
+    if (master) riemann_rot_vjp<SOLVER>(gas, So, Sn, lFr, ao, an);
+    else riemann_rot_vjp<SOLVER>(gas, Sn, So, lFr, an, ao);
+    double gOwn[5], gNb[5];
+    rot_state_adjoint(Uo, gas, nv, t1, t2, ao, gOwn);
+    rot_state_adjoint(Un, gas, nv, t1, t2, an, gNb);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      sLf[v * NF + x] = gOwn[v];
+      XU[5 * (size_t)own + v] = gNb[v];
+    }
+#endif
   }
 
   // split-flux adjoint: each pair (i,l) contributes J_s^T (DV(l,i) mu_i + DV(i,l) mu_l) to both
@@ -195,6 +461,7 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict
 template <int N>
 __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict__ U, DevMesh m,
                                                      const double* __restrict__ FUa, const double* __restrict__ lam,
+                                                     const double* __restrict__ ub, double cub,
                                                      const double* __restrict__ Xg, Gas gas, double* __restrict__ out,
                                                      double* __restrict__ Xq) {
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
@@ -215,7 +482,8 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict_
     sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
     const double sj = m.sJ[gid];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sMu[c * np + t] = -sj * lam[5 * gid + 1 + c];
+    for (int c = 0; c < 4; ++c)
+      sMu[c * np + t] = -sj * (lam[5 * gid + 1 + c] + (ub ? cub * ub[5 * gid + 1 + c] : 0.0));
   }
   for (int x = t; x < NF; x += NT) {
     const unsigned own = face_slot<N>(m, e, x);
@@ -368,6 +636,52 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_gather(const double* __restric
   for (int v = 0; v < 5; ++v) out[5 * gid + v] += lU[v] + lP[v];
 }
 
+// One reverse LSERK stage's register updates, with the neighbours' face parts of the adjoint:
+// lam = rbar + cub*ubar (the adjoint the stage's right-hand side saw), then
+// ubar += out + face parts,  rbar = a*lam  (a = -RKA of the stage, 0 for the first).
+template <int N>
+__global__ __launch_bounds__(D<N>::NT) void k_adj_stage_update(const double* __restrict__ U, DevMesh m,
+                                                             const double* __restrict__ XU,
+                                                             const double* __restrict__ Xq, Gas gas,
+                                                             const double* __restrict__ out, double* __restrict__ ub,
+                                                             double* __restrict__ rbar, double cub, double a) {
+  constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
+  __shared__ unsigned sIdx[NF];
+  const int e = blockIdx.x, t = threadIdx.x;
+  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot<N>(m, e, x);
+  __syncthreads();
+  if (t >= np) return;
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  const int sides = node_sides<N>(i, j, k);
+  const size_t gid = (size_t)np * e + t;
+  double lU[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) lU[v] = out[5 * gid + v];
+  if (sides) {
+    const Prim q = cons_to_prim(U + 5 * gid, gas);
+    double lq[4] = {0, 0, 0, 0};
+#pragma unroll 1
+    for (int ls = 1; ls <= 6; ++ls) {
+      if (!(sides & (1 << (ls - 1)))) continue;
+      const unsigned nb = slot_nbr<N>(sIdx[face_point<N>(ls, i, j, k)]);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) lU[v] += XU[5 * (size_t)nb + v];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) lq[v] += Xq[4 * (size_t)nb + v];
+    }
+    double lP[5];
+    prim_adjoint(q, U[5 * gid + 4], gas, 0.0, lq[0], lq[1], lq[2], 0.0, lq[3], 0.0, lP);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) lU[v] += lP[v];
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double u0 = ub[5 * gid + v];
+    rbar[5 * gid + v] = a * (rbar[5 * gid + v] + cub * u0);
+    ub[5 * gid + v] = u0 + lU[v];
+  }
+}
+
 // y += a*x and y = a*y over n doubles (the LSERK adjoint's register updates)
 __global__ __launch_bounds__(256) void k_axpy(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,37 +692,58 @@ __global__ __launch_bounds__(256) void k_scale(size_t n, double a, double* __res
   if (i < n) y[i] *= a;
 }
 
+// The VJP at U of lam (+ cub*ub when ub is given). stage == nullptr: out = result. Otherwise a
+// reverse LSERK stage: stage[0] = a, and ub += result, rbar(=lam) = a*(lam + cub*ub) in place.
 template <int N>
 hipError_t vjp_n(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                 const double* lam, double* out, hipStream_t s) {
+                 const double* lam, const double* ub, double cub, double* out, hipStream_t s, const double* stage) {
   const int K = m.nElems;
   constexpr int NT = D<N>::NT;
   hipError_t err = launch_faces_of(m, U, w.FUa, s);
   if (err != hipSuccess) return err;
   if (riemann == 1)
-    hipLaunchKernelGGL((k_adj_euler<N, 1>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, gas, out, w.XU, w.Xg);
+    hipLaunchKernelGGL((k_adj_euler<N, 1>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, gas, out, w.XU, w.Xg);
   else if (riemann == 3)
-    hipLaunchKernelGGL((k_adj_euler<N, 3>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, gas, out, w.XU, w.Xg);
+    hipLaunchKernelGGL((k_adj_euler<N, 3>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, gas, out, w.XU, w.Xg);
   else
-    hipLaunchKernelGGL((k_adj_euler<N, 33>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, gas, out, w.XU, w.Xg);
-  hipLaunchKernelGGL((k_adj_visc<N>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, w.Xg, gas, out, w.Xq);
-  hipLaunchKernelGGL((k_adj_gather<N>), dim3(K), dim3(NT), 0, s, U, m, w.XU, w.Xq, gas, out);
+    hipLaunchKernelGGL((k_adj_euler<N, 33>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, gas, out, w.XU,
+                       w.Xg);
+  hipLaunchKernelGGL((k_adj_visc<N>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, w.Xg, gas, out, w.Xq);
+  if (!stage)
+    hipLaunchKernelGGL((k_adj_gather<N>), dim3(K), dim3(NT), 0, s, U, m, w.XU, w.Xq, gas, out);
+  else
+    hipLaunchKernelGGL((k_adj_stage_update<N>), dim3(K), dim3(NT), 0, s, U, m, w.XU, w.Xq, gas, (const double*)out,
+                       const_cast<double*>(ub), const_cast<double*>(lam), cub, stage[0]);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                          const double* lam, double* out, hipStream_t s) {
+namespace {
+hipError_t vjp_dispatch(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
+                        const double* lam, const double* ub, double cub, double* out, hipStream_t s,
+                        const double* stage) {
   switch (m.N) {
-    case 2: return vjp_n<2>(m, w, gas, riemann, U, lam, out, s);
-    case 3: return vjp_n<3>(m, w, gas, riemann, U, lam, out, s);
-    case 4: return vjp_n<4>(m, w, gas, riemann, U, lam, out, s);
-    case 5: return vjp_n<5>(m, w, gas, riemann, U, lam, out, s);
-    case 6: return vjp_n<6>(m, w, gas, riemann, U, lam, out, s);
-    case 7: return vjp_n<7>(m, w, gas, riemann, U, lam, out, s);
+    case 2: return vjp_n<2>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
+    case 3: return vjp_n<3>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
+    case 4: return vjp_n<4>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
+    case 5: return vjp_n<5>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
+    case 6: return vjp_n<6>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
+    case 7: return vjp_n<7>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
     default: return hipErrorInvalidValue;
   }
+}
+}  // namespace
+
+hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
+                          const double* lam, double* out, hipStream_t s) {
+  return vjp_dispatch(m, w, gas, riemann, U, lam, nullptr, 0.0, out, s, nullptr);
+}
+
+hipError_t launch_adjoint_stage(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
+                                double* rbar, double* ubar, double cub, double a, double* tmp, hipStream_t s) {
+  const double stage[1] = {a};
+  return vjp_dispatch(m, w, gas, riemann, U, rbar, ubar, cub, tmp, s, stage);
 }
 
 hipError_t launch_axpy(size_t n, double a, const double* x, double* y, hipStream_t s) {

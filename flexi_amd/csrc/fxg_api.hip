@@ -560,19 +560,26 @@ int fxg_rhs_vjp_dev(fxg_ctx* c, double t, const double* U, const double* lam, do
 }
 
 int fxg_forward_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, const double* RKA, const double* RKb,
-                        const double* RKc, double* U, void* stream) {
+                        const double* RKc, double* U, double* ckpt, int checkpointEvery, void* stream) {
   (void)RKc;
   if (!c || !dt || !U) return fail("NULL argument");
   if (nSteps < 0) return fail("nSteps must not be negative");
+  if (ckpt && checkpointEvery < 1) return fail("checkpointEvery must be positive");
   if (check_rk(nStages, RKA, RKb)) return 1;
   const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  for (int it = 0; it < nSteps; ++it)
+  const size_t vol5 = (size_t)5 * c->np * c->K;
+  for (int it = 0; it < nSteps; ++it) {
+    if (ckpt && it % checkpointEvery == 0)
+      HIPCHECK(hipMemcpyAsync(ckpt + (size_t)(it / checkpointEvery) * vol5, U, vol5 * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
     if (lserk(c, dt[it], nStages, RKA, RKb, U, c->R, s)) return 1;
+  }
   return 0;
 }
 
 int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, const double* RKA, const double* RKb,
-                        const double* RKc, const double* U0, double* ubar, int checkpointEvery, void* stream) {
+                        const double* RKc, const double* U0, const double* ckptIn, double* ubar, int checkpointEvery,
+                        void* stream) {
   (void)RKc;
   if (!c || !dt || !U0 || !ubar) return fail("NULL argument");
   if (nSteps < 0) return fail("nSteps must not be negative");
@@ -585,17 +592,23 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
   const int every = checkpointEvery < nSteps ? checkpointEvery : nSteps;
   const int nck = (nSteps + every - 1) / every;
   DevBuffers buf;
-  double* ck = buf.get((size_t)nck * vol5);
+  const double* ck = ckptIn;
   double* seg = buf.get((size_t)every * nStages * vol5);
   double* Uw = buf.get(vol5);
   double* R = buf.get(vol5);
   double* rbar = buf.get(vol5);
-  if (!ck || !seg || !Uw || !R || !rbar) return fail("fxg_adjoint_n_steps: out of device memory for checkpoints");
-  // forward sweep with a checkpoint of U every `every` steps
-  HIPCHECK(hipMemcpyAsync(Uw, U0, bytes, hipMemcpyDeviceToDevice, s));
-  for (int it = 0; it < nSteps; ++it) {
-    if (it % every == 0) HIPCHECK(hipMemcpyAsync(ck + (size_t)(it / every) * vol5, Uw, bytes, hipMemcpyDeviceToDevice, s));
-    if (lserk(c, dt[it], nStages, RKA, RKb, Uw, R, s)) return 1;
+  if (!seg || !Uw || !R || !rbar) return fail("fxg_adjoint_n_steps: out of device memory for the stage states");
+  if (!ck) {
+    // forward sweep with a checkpoint of U every `every` steps
+    double* ckw = buf.get((size_t)nck * vol5);
+    if (!ckw) return fail("fxg_adjoint_n_steps: out of device memory for checkpoints");
+    HIPCHECK(hipMemcpyAsync(Uw, U0, bytes, hipMemcpyDeviceToDevice, s));
+    for (int it = 0; it < nSteps; ++it) {
+      if (it % every == 0)
+        HIPCHECK(hipMemcpyAsync(ckw + (size_t)(it / every) * vol5, Uw, bytes, hipMemcpyDeviceToDevice, s));
+      if (lserk(c, dt[it], nStages, RKA, RKb, Uw, R, s)) return 1;
+    }
+    ck = ckw;
   }
   // reverse sweep, one segment at a time: recompute the stage states of the segment from its
   // checkpoint, then run the adjoint of each stage backwards (R_s = Ut_s - RKA_s R_{s-1},
@@ -613,10 +626,8 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
     for (int it = s1 - 1; it >= s0; --it)
       for (int st = nStages - 1; st >= 0; --st) {
         const double* Us = seg + ((size_t)(it - s0) * nStages + st) * vol5;
-        HIPCHECK(fxg::launch_axpy(vol5, RKb[st] * dt[it], ubar, rbar, s));
-        HIPCHECK(fxg::launch_rhs_vjp(c->m, c->aw, c->gas, c->p.riemann, Us, rbar, c->adjTmp, s));
-        HIPCHECK(fxg::launch_axpy(vol5, 1.0, c->adjTmp, ubar, s));
-        HIPCHECK(fxg::launch_scale(vol5, st == 0 ? 0.0 : -RKA[st - 1], rbar, s));
+        HIPCHECK(fxg::launch_adjoint_stage(c->m, c->aw, c->gas, c->p.riemann, Us, rbar, ubar, RKb[st] * dt[it],
+                                           st == 0 ? 0.0 : -RKA[st - 1], c->adjTmp, s));
       }
   }
   HIPCHECK(hipStreamSynchronize(s));

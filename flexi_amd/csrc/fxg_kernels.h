@@ -114,6 +114,10 @@ struct AdjointWork {
 // out = (dUt/dU)^T lam at U (FLEXI layout, FP64, Gauss-Lobatto split form, N 2..7).
 hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
                           const double* lam, double* out, hipStream_t s);
+// One reverse LSERK stage at the stage state U: lam = rbar + cub*ubar; ubar += (dUt/dU)^T lam;
+// rbar = a*lam. `tmp` is a state-sized scratch array.
+hipError_t launch_adjoint_stage(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
+                                double* rbar, double* ubar, double cub, double a, double* tmp, hipStream_t s);
 hipError_t launch_axpy(size_t n, double a, const double* x, double* y, hipStream_t s);
 hipError_t launch_scale(size_t n, double a, double* y, hipStream_t s);
 

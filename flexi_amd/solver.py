@@ -176,19 +176,23 @@ class DGSolver:
         dts = np.ascontiguousarray(np.asarray(dts, dtype=np.float64))
         return dts, dts.ctypes.data_as(C.POINTER(C.c_double))
 
-    def forward_n_steps_dev(self, U_ptr: int, dts, stream: int = 0) -> None:
-        """len(dts) LSERK steps on the device state at U_ptr with the given time steps."""
+    def forward_n_steps_dev(self, U_ptr: int, dts, ckpt_ptr: int = 0, checkpoint_every: int = 20,
+                            stream: int = 0) -> None:
+        """len(dts) LSERK steps on the device state at U_ptr with the given time steps; with
+        ckpt_ptr, the state before every checkpoint_every-th step is stored there."""
         arr, p = self._dt_array(dts)
         _lib.check(_lib.lib().fxg_forward_n_steps(self.ctx, len(arr), p, self.scheme.nStages, self._RKA, self._RKb,
-                                                  self._RKc, U_ptr, stream or None))
+                                                  self._RKc, U_ptr, ckpt_ptr or None, int(checkpoint_every),
+                                                  stream or None))
 
-    def adjoint_n_steps_dev(self, U0_ptr: int, ubar_ptr: int, dts, checkpoint_every: int = 20,
+    def adjoint_n_steps_dev(self, U0_ptr: int, ubar_ptr: int, dts, checkpoint_every: int = 20, ckpt_ptr: int = 0,
                             stream: int = 0) -> None:
-        """ubar <- (dU(T)/dU(0))^T ubar for the steps of forward_n_steps_dev from U0."""
+        """ubar <- (dU(T)/dU(0))^T ubar for the steps of forward_n_steps_dev from U0 (from the
+        checkpoints at ckpt_ptr when given)."""
         arr, p = self._dt_array(dts)
         _lib.check(_lib.lib().fxg_adjoint_n_steps(self.ctx, len(arr), p, self.scheme.nStages, self._RKA, self._RKb,
-                                                  self._RKc, U0_ptr, ubar_ptr, int(checkpoint_every),
-                                                  stream or None))
+                                                  self._RKc, U0_ptr, ckpt_ptr or None, ubar_ptr,
+                                                  int(checkpoint_every), stream or None))
 
     def kernel_pipeline(self) -> str:
         """'fused' or 'classic' (which stage kernels this context launches)."""

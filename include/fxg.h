@@ -139,18 +139,22 @@ int fxg_timestep_lserkw2_dev(fxg_ctx* ctx, double t, double dt, int nStages, con
  *   DGTimeDerivative_weakForm at U applied to lam (asynchronous on `stream`, NULL = the context's).
  * fxg_forward_n_steps: nSteps calls of TimeStepByLSERKW2 on U_dev with the time steps dt[0..nSteps-1]
  *   (host array; the sequence a primal run chose with CALCTIMESTEP, frozen so that the adjoint
- *   differentiates the same map).
+ *   differentiates the same map). With ckpt_dev (may be NULL) the state before step k is copied to
+ *   ckpt_dev[k/checkpointEvery] for every k that checkpointEvery divides (ceil(nSteps/every) states).
  * fxg_adjoint_n_steps: ubar_dev holds dJ/dU(T) on entry and dJ/dU(0) on return, for the map
- *   U(0) -> U(T) of fxg_forward_n_steps started from U0_dev. The states are recomputed from
- *   checkpoints taken every checkpointEvery steps (device memory: ceil(nSteps/checkpointEvery)
- *   + checkpointEvery*nStages + 3 states, allocated for the call). Synchronous. */
+ *   U(0) -> U(T) of fxg_forward_n_steps started from U0_dev. The stage states are recomputed
+ *   segment by segment from the checkpoints: those of a forward call with the same
+ *   checkpointEvery (ckpt_dev), or, with ckpt_dev NULL, from a forward sweep of its own. Device
+ *   memory allocated for the call: checkpointEvery*nStages + 3 states (+ the checkpoints when it
+ *   takes them itself). Synchronous. */
 int fxg_rhs_vjp_dev(fxg_ctx* ctx, double t, const double* U_dev, const double* lam_dev, double* out_dev,
                     void* stream);
 int fxg_forward_n_steps(fxg_ctx* ctx, int nSteps, const double* dt, int nStages, const double* RKA,
-                        const double* RKb, const double* RKc, double* U_dev, void* stream);
-int fxg_adjoint_n_steps(fxg_ctx* ctx, int nSteps, const double* dt, int nStages, const double* RKA,
-                        const double* RKb, const double* RKc, const double* U0_dev, double* ubar_dev,
+                        const double* RKb, const double* RKc, double* U_dev, double* ckpt_dev,
                         int checkpointEvery, void* stream);
+int fxg_adjoint_n_steps(fxg_ctx* ctx, int nSteps, const double* dt, int nStages, const double* RKA,
+                        const double* RKb, const double* RKc, const double* U0_dev, const double* ckpt_dev,
+                        double* ubar_dev, int checkpointEvery, void* stream);
 /* Which stage kernels the context runs: "fused" (Gauss-Lobatto split form, N 2..7: two
  * kernels per stage, face states carried between LSERK stages) or "classic" (prolongation,
  * lifting, side fluxes, integrals). No counterpart in the reference; FXG_PIPELINE=classic in
