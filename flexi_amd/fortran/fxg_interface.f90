@@ -36,6 +36,7 @@ PUBLIC :: fxg_last_error, fxg_version, fxg_create, fxg_create_from_nodes, fxg_de
 PUBLIC :: fxg_set_state, fxg_get_state, fxg_get_ut, fxg_get_gradients, fxg_get_metrics
 PUBLIC :: fxg_dg_time_derivative, fxg_timestep_lserkw2, fxg_calc_timestep
 PUBLIC :: fxg_dg_time_derivative_dev, fxg_timestep_lserkw2_dev, fxg_pipeline, fxg_sync, fxg_sizes
+PUBLIC :: fxg_rhs_vjp_dev, fxg_forward_n_steps, fxg_adjoint_n_steps
 PUBLIC :: fxg_profile_enable, fxg_profile_read, fxg_string
 
 INTERFACE
@@ -123,6 +124,25 @@ INTERFACE
     REAL(C_DOUBLE),VALUE      :: t, dt
     INTEGER(C_INT),VALUE      :: nStages
     REAL(C_DOUBLE),INTENT(IN) :: RKA(*), RKb(*), RKc(*)
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION fxg_rhs_vjp_dev(ctx,t,U_dev,lam_dev,out_dev,stream) BIND(C,NAME='fxg_rhs_vjp_dev')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR),VALUE    :: ctx, U_dev, lam_dev, out_dev, stream
+    REAL(C_DOUBLE),VALUE :: t
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION fxg_forward_n_steps(ctx,nSteps,dt,nStages,RKA,RKb,RKc,U_dev,ckpt_dev,checkpointEvery,stream) &
+      BIND(C,NAME='fxg_forward_n_steps')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR),VALUE         :: ctx, U_dev, ckpt_dev, stream
+    INTEGER(C_INT),VALUE      :: nSteps, nStages, checkpointEvery
+    REAL(C_DOUBLE),INTENT(IN) :: dt(*), RKA(*), RKb(*), RKc(*)
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION fxg_adjoint_n_steps(ctx,nSteps,dt,nStages,RKA,RKb,RKc,U0_dev,ckpt_dev,ubar_dev, &
+      checkpointEvery,stream) BIND(C,NAME='fxg_adjoint_n_steps')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR),VALUE         :: ctx, U0_dev, ckpt_dev, ubar_dev, stream
+    INTEGER(C_INT),VALUE      :: nSteps, nStages, checkpointEvery
+    REAL(C_DOUBLE),INTENT(IN) :: dt(*), RKA(*), RKb(*), RKc(*)
   END FUNCTION
   FUNCTION fxg_pipeline(ctx) BIND(C,NAME='fxg_pipeline')
     IMPORT :: C_PTR
