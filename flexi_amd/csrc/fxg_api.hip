@@ -154,6 +154,15 @@ struct fxg_ctx {
   bool fused = false;
   fxg::FusedWork fw{};
   int cur = 0;
+  // the state whose face values FU[cur] holds (only trusted for the context's own state, which
+  // changes only through fxg_set_state and the library's own updates)
+  const double* faces_of = nullptr;
+  // After a fused LSERK step on the context's state, Ut and the lifted gradients of its last
+  // right-hand side (at the state before the last update, U - RKb_last*dt*R) are recomputed on
+  // request: the fused stages do not store them.
+  bool rhs_stale = false;
+  double last_b_dt = 0.0;
+  double* Uprev = nullptr;
   // discrete adjoint scratch, allocated on first use
   fxg::AdjointWork aw{};
   double* adjTmp = nullptr;
@@ -297,6 +306,7 @@ int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStre
     HIPCHECK(fxg::launch_rhs_fused(c->m, c->fw, c->gas, c->p.riemann, c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st,
                                    faces_fresh, c->cur, st.mode == 0 ? c->w.G : nullptr, s, c->timer));
     if (st.mode != 0) c->cur ^= 1;
+    c->faces_of = U;  // FU[cur] now holds the face values of U (updated in place in RK modes)
     return 0;
   }
   HIPCHECK(fxg::launch_rhs(c->m, c->w, c->gas, c->p.nodeType, c->p.splitDG == FXG_SPLITDG_PI, c->p.riemann,
@@ -311,13 +321,15 @@ int lserk_stage(fxg_ctx* c, double dt, int st, const double* RKA, const double* 
   return rhs(c, U, nullptr, stage, s, st > 0);
 }
 
+// One TimeStepByLSERKW2 step. `faces_known`: FU[cur] already holds the face values of U (the
+// previous step of the same call, or of the context's own state), so the first stage reuses them.
 int lserk(fxg_ctx* c, double dt, int nStages, const double* RKA, const double* RKb, double* U, double* R,
-          hipStream_t s) {
+          hipStream_t s, bool faces_known = false) {
   if (nStages < 1) return fail("nStages must be positive");
   if (!RKb || (nStages > 1 && !RKA)) return fail("RK coefficient arrays are NULL");
   for (int st = 0; st < nStages; ++st) {
     fxg::RKStage stage{st == 0 ? 1 : 2, st == 0 ? 0.0 : -RKA[st - 1], RKb[st] * dt, U, R};
-    if (rhs(c, U, nullptr, stage, s, st > 0)) return 1;
+    if (rhs(c, U, nullptr, stage, s, st > 0 || faces_known)) return 1;
   }
   return 0;
 }
@@ -353,6 +365,21 @@ struct DevBuffers {
     return d;
   }
 };
+
+// Ut and gradients of the last right-hand side of a fused LSERK step on the context's state:
+// the stage input was U_{s-1} = U - RKb_s*dt*R (to round-off), and one right-hand side there
+// restores what the four-kernel path leaves in Ut and the gradient array.
+int refresh_last_rhs(fxg_ctx* c) {
+  if (!c->rhs_stale) return 0;
+  const size_t vol5 = (size_t)5 * c->np * c->K;
+  if (!c->Uprev && alloc_owned(c, &c->Uprev, vol5)) return 1;
+  HIPCHECK(hipMemcpyAsync(c->Uprev, c->U, vol5 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(fxg::launch_axpy(vol5, -c->last_b_dt, c->R, c->Uprev, c->stream));
+  fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
+  if (rhs(c, c->Uprev, c->Ut, st, c->stream)) return 1;
+  c->rhs_stale = false;
+  return 0;
+}
 
 }  // namespace
 
@@ -432,6 +459,8 @@ int fxg_sizes(fxg_ctx* c, int* N, int* K, int* S) {
 
 int fxg_set_state(fxg_ctx* c, const double* U) {
   if (!c || !U) return fail("NULL argument");
+  c->faces_of = nullptr;
+  c->rhs_stale = false;
   HIPCHECK(hipMemcpyAsync(c->U, U, 5 * (size_t)c->np * c->K * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
@@ -446,6 +475,7 @@ int fxg_get_state(fxg_ctx* c, double* U) {
 
 int fxg_get_ut(fxg_ctx* c, double* Ut) {
   if (!c || !Ut) return fail("NULL argument");
+  if (refresh_last_rhs(c)) return 1;
   HIPCHECK(hipMemcpyAsync(Ut, c->Ut, 5 * (size_t)c->np * c->K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
@@ -453,6 +483,7 @@ int fxg_get_ut(fxg_ctx* c, double* Ut) {
 
 int fxg_get_gradients(fxg_ctx* c, double* gx, double* gy, double* gz) {
   if (!c || !gx || !gy || !gz) return fail("NULL argument");
+  if (refresh_last_rhs(c)) return 1;
   const size_t nn = 4 * (size_t)c->np * c->K;
   double* tmp = nullptr;
   if (dev_alloc(&tmp, 3 * nn)) return 1;
@@ -484,6 +515,7 @@ int fxg_dg_time_derivative(fxg_ctx* c, double t) {
   (void)t;  // no time-dependent sources or boundaries on periodic meshes
   if (!c) return fail("ctx is NULL");
   fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
+  c->rhs_stale = false;
   if (rhs(c, c->U, c->Ut, st, c->stream)) return 1;
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
@@ -494,7 +526,9 @@ int fxg_timestep_lserkw2(fxg_ctx* c, double t, double dt, int nStages, const dou
   (void)t;
   (void)RKc;  // stage times only enter time-dependent sources, which periodic flows do not have
   if (!c) return fail("ctx is NULL");
-  if (lserk(c, dt, nStages, RKA, RKb, c->U, c->R, c->stream)) return 1;
+  if (lserk(c, dt, nStages, RKA, RKb, c->U, c->R, c->stream, c->fused && c->faces_of == c->U)) return 1;
+  c->rhs_stale = c->fused;
+  c->last_b_dt = RKb[nStages - 1] * dt;
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -503,7 +537,9 @@ int fxg_dg_time_derivative_dev(fxg_ctx* c, double t, const double* U, double* Ut
   (void)t;
   if (!c || !U || !Ut) return fail("NULL argument");
   fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
-  return rhs(c, U, Ut, st, stream ? (hipStream_t)stream : c->stream);
+  const int rc = rhs(c, U, Ut, st, stream ? (hipStream_t)stream : c->stream);
+  c->faces_of = nullptr;  // the caller's buffer may change before the next call
+  return rc;
 }
 
 int fxg_timestep_lserkw2_dev(fxg_ctx* c, double t, double dt, int nStages, const double* RKA, const double* RKb,
@@ -511,7 +547,9 @@ int fxg_timestep_lserkw2_dev(fxg_ctx* c, double t, double dt, int nStages, const
   (void)t;
   (void)RKc;
   if (!c || !U || !R) return fail("NULL argument");
-  return lserk(c, dt, nStages, RKA, RKb, U, R, stream ? (hipStream_t)stream : c->stream);
+  const int rc = lserk(c, dt, nStages, RKA, RKb, U, R, stream ? (hipStream_t)stream : c->stream);
+  c->faces_of = nullptr;
+  return rc;
 }
 
 const char* fxg_pipeline(fxg_ctx* c) { return !c ? "" : (c->fused ? "fused" : "classic"); }
@@ -572,8 +610,12 @@ int fxg_forward_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
     if (ckpt && it % checkpointEvery == 0)
       HIPCHECK(hipMemcpyAsync(ckpt + (size_t)(it / checkpointEvery) * vol5, U, vol5 * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
-    if (lserk(c, dt[it], nStages, RKA, RKb, U, c->R, s)) return 1;
+    if (lserk(c, dt[it], nStages, RKA, RKb, U, c->R, s, it > 0 && c->fused)) {
+      c->faces_of = nullptr;
+      return 1;
+    }
   }
+  c->faces_of = nullptr;
   return 0;
 }
 
@@ -586,6 +628,7 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
   if (checkpointEvery < 1) return fail("checkpointEvery must be positive");
   if (check_rk(nStages, RKA, RKb)) return 1;
   if (ensure_adjoint(c)) return 1;
+  c->faces_of = nullptr;  // the recomputation below runs on scratch states
   if (nSteps == 0) return 0;
   const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const size_t vol5 = (size_t)5 * c->np * c->K, bytes = vol5 * sizeof(double);
@@ -631,6 +674,7 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
       }
   }
   HIPCHECK(hipStreamSynchronize(s));
+  c->faces_of = nullptr;
   return 0;
 }
 

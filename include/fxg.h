@@ -100,11 +100,11 @@ int fxg_destroy(fxg_ctx* ctx);
 int fxg_set_state(fxg_ctx* ctx, const double* U);
 int fxg_get_state(fxg_ctx* ctx, double* U);
 int fxg_get_ut(fxg_ctx* ctx, double* Ut);
-/* Lifted gradients of the last fxg_dg_time_derivative(_dev) call,
- * gradU{x,y,z}(4,0:N,0:N,0:N,nElems) for the lifting variables (u,v,w,T)
- * (src/dg/lifting/lifting_vars.f90; PP_OPTLIFT=1 set, eos.h:114). LSERK stages do not store
- * them; FLEXI's analysis evaluates a fresh right-hand side before it reads them
- * (src/timedisc/timedisc_func.f90:324-325). */
+/* Lifted gradients of the last right-hand side, gradU{x,y,z}(4,0:N,0:N,0:N,nElems) for the
+ * lifting variables (u,v,w,T) (src/dg/lifting/lifting_vars.f90; PP_OPTLIFT=1 set, eos.h:114):
+ * after fxg_dg_time_derivative those of U; after fxg_timestep_lserkw2, as in FLEXI, those of the
+ * last stage's right-hand side (at the state before the last update; the fused Gauss-Lobatto
+ * path recomputes them there on request, equal to round-off). fxg_get_ut likewise. */
 int fxg_get_gradients(fxg_ctx* ctx, double* gx, double* gy, double* gz);
 /* Geometry as used on the device (for checking fxg_create_from_nodes): same layouts as fxg_mesh
  * plus Elem_xGP(3,0:N,0:N,0:N,nElems). Any pointer may be NULL. */

@@ -78,3 +78,78 @@ def test_curvilinear_freestream_24cubed_n5_gpu_metrics(pipeline, monkeypatch):
     # FLEXI and gives the slave the master's NormVec*SurfElem, which differs from the slave's own
     # boundary metric by the round-off of two separate metric evaluations (~2e-12 here).
     assert err <= (1e-13 if pipeline == "fused" else 1e-11)
+
+
+def test_gpu_reproduces_flexi_tgv_split_regression_trajectory():
+    """regressioncheck/checks/tgv/split of the reference: N=7 GL, split PI, RoeEntropyFix, BR1,
+    8^3 on [0,2pi]^3, CFL=DFL=0.8, TGV analysis every 10 steps. The GPU state and lifted
+    gradients are analysed with the oracle's AnalyzeTestcase restatement and compared with the
+    reference CSV over the first 50 steps (the reference's own tolerance is 1e-4 relative). The
+    gradients are those of the last stage's right-hand side, which the library recomputes from
+    U - RKb_5*dt*R for the fused path."""
+    import flexi_fixtures as fx
+    import oracle as orc
+    from flexi_amd.solver import DGSolver
+    header, data = fx.tgv_split_csv()
+    mesh = periodic_box(8, 8, 8, lo=(0, 0, 0))
+    m = orc.calc_metrics(7, orc.NODE_GL, mesh)
+    S = orc.Solver(orc.params(7, orc.NODE_GL, 1, orc.RIEMANN_ROEENTROPYFIX, mu0=6.25e-4, R=71.42857), mesh, m)
+    g = DGSolver(mesh, 7, node_type="gauss-lobatto", split_dg="pi", riemann="roeentropyfix", metrics=m,
+                 mu0=6.25e-4, R=71.42857, CFLScale=0.8, DFLScale=0.8)
+    assert g.kernel_pipeline() == "fused"
+    g.set_state(testcase.tgv(m["Elem_xGP"]))
+
+    def check(row, t):
+        # gradients of the last right-hand side, as FLEXI's analysis every nAnalyzeTestCase
+        # iterations reads them (timedisc_func.f90:340)
+        out = S.tgv_analyze(g.get_state(), g.gradients())
+        worst = 0.0
+        for c, (got, ref) in enumerate(zip(out, row[1:])):
+            if abs(ref) < 1e-6:
+                assert abs(got - ref) < 1e-12, header[c + 1]
+            else:
+                worst = max(worst, abs(got - ref) / abs(ref))
+        print(f"t={t:.6f}: max relative deviation from the reference CSV {worst:.2e}")
+        assert worst <= 1e-8
+
+    t = 0.0
+    g.DGTimeDerivative_weakForm(t)
+    check(data[0], t)
+    for it in range(1, 51):
+        dt, err = g.CalcTimeStep()
+        assert err == 0
+        g.dt = dt
+        g.TimeStepByLSERKW2(t)
+        t += dt
+        if it % 10 == 0:
+            row = data[it // 10]
+            assert abs(t - row[0]) <= 1e-12 * row[0]
+            check(row, t)
+
+
+def test_plain_method_50_steps_against_oracle():
+    """TGV on [-pi,pi]^3, 8^3 elements, N=3 Gauss, weak form, LF, BR1, CarpenterRK4-5, CFL=DFL=0.9,
+    50 steps with CALCTIMESTEP: GPU (four-kernel path) against the oracle step by step."""
+    import oracle as orc
+    from flexi_amd.solver import DGSolver
+    mesh = periodic_box(8, 8, 8)
+    m = orc.calc_metrics(3, orc.NODE_G, mesh)
+    S = orc.Solver(orc.params(3, orc.NODE_G, 0, orc.RIEMANN_LF), mesh, m)
+    g = DGSolver(mesh, 3, node_type="gauss", split_dg=None, riemann="lf", metrics=m, CFLScale=0.9, DFLScale=0.9)
+    U0 = testcase.tgv(m["Elem_xGP"])
+    Uo = np.asfortranarray(U0.copy())
+    R = orc.F(U0.shape)
+    g.set_state(U0)
+    t = 0.0
+    for it in range(50):
+        dt, err = g.CalcTimeStep()
+        dto, erro, _ = S.calc_timestep(Uo, g.CFL, g.DFL)
+        assert err == 0 and erro == 0 and abs(dt - dto) <= 1e-13 * dto
+        g.dt = dt
+        g.TimeStepByLSERKW2(t)
+        S.step(Uo, t, dt, R)
+        t += dt
+    Ug = g.get_state()
+    err = np.linalg.norm((Ug - Uo).ravel()) / np.linalg.norm((Uo - U0).ravel())
+    print(f"50 steps to t={t:.4f}: ||U_gpu - U_oracle|| / ||U - U0|| = {err:.2e}")
+    assert err <= 1e-10

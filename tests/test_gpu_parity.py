@@ -190,3 +190,27 @@ def test_fused_rhs_conserves_on_periodic_mesh():
     for v in range(5):
         # the two sides of a face take n*SurfElem from their own metrics, equal to round-off
         assert abs(np.sum(W * Ut[v])) <= 1e-13 * np.sum(W * np.abs(Ut[v]))
+
+
+def test_fused_steps_reuse_face_values_between_calls():
+    # consecutive TimeStepByLSERKW2 calls on the context's state reuse the face values written by
+    # the last stage; set_state in between must invalidate them
+    mesh, m, oracle, gpu, U = setup(5, orc.NODE_GL, True, orc.RIEMANN_ROE, shape=(2, 3, 2))
+    dt = 2e-3
+    Uo = np.asfortranarray(U.copy())
+    R = orc.F(U.shape)
+    for it in range(3):
+        oracle.step(Uo, it * dt, dt, R)
+    gpu.set_state(U)
+    gpu.dt = dt
+    gpu.DGTimeDerivative_weakForm(0.0)   # faces of U from an RHS call
+    for it in range(3):
+        gpu.CalcTimeStep()                # does not change U
+        gpu.TimeStepByLSERKW2(it * dt)
+    assert rel(gpu.get_state() - U, Uo - U) <= 1e-11
+    U2 = np.asfortranarray(U * 1.01)
+    gpu.set_state(U2)                     # new state: stale faces must not be used
+    gpu.TimeStepByLSERKW2(0.0)
+    U2o = np.asfortranarray(U2.copy())
+    oracle.step(U2o, 0.0, dt, orc.F(U.shape))
+    assert rel(gpu.get_state() - U2, U2o - U2) <= 1e-11
