@@ -163,6 +163,10 @@ struct fxg_ctx {
   bool rhs_stale = false;
   double last_b_dt = 0.0;
   double* Uprev = nullptr;
+  // CALCTIMESTEP spectral-radius sums per element, computed by a fused last stage for lam_of
+  double* lamSum = nullptr;
+  int* lamNan = nullptr;
+  const double* lam_of = nullptr;
   // discrete adjoint scratch, allocated on first use
   fxg::AdjointWork aw{};
   double* adjTmp = nullptr;
@@ -269,7 +273,8 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   if (c->fused) {
     char* FG;
     if (alloc_owned(c, &c->fw.FU[0], 10 * face) || alloc_owned(c, &c->fw.FU[1], 10 * face) ||
-        alloc_owned(c, &FG, 8 * face * gsz) || alloc_owned(c, &c->fw.Jsoa, 10 * vol))
+        alloc_owned(c, &FG, 8 * face * gsz) || alloc_owned(c, &c->fw.Jsoa, 10 * vol) ||
+        alloc_owned(c, &c->lamSum, 2 * (size_t)K) || alloc_owned(c, &c->lamNan, (size_t)K))
       return 1;
     c->fw.FG = FG;
   }
@@ -323,12 +328,18 @@ int lserk_stage(fxg_ctx* c, double dt, int st, const double* RKA, const double* 
 
 // One TimeStepByLSERKW2 step. `faces_known`: FU[cur] already holds the face values of U (the
 // previous step of the same call, or of the context's own state), so the first stage reuses them.
+// `want_lambda`: the fused last stage also stores CALCTIMESTEP's per-element spectral radii of the
+// new state (used by the next fxg_calc_timestep on the context's state).
 int lserk(fxg_ctx* c, double dt, int nStages, const double* RKA, const double* RKb, double* U, double* R,
-          hipStream_t s, bool faces_known = false) {
+          hipStream_t s, bool faces_known = false, bool want_lambda = false) {
   if (nStages < 1) return fail("nStages must be positive");
   if (!RKb || (nStages > 1 && !RKA)) return fail("RK coefficient arrays are NULL");
   for (int st = 0; st < nStages; ++st) {
     fxg::RKStage stage{st == 0 ? 1 : 2, st == 0 ? 0.0 : -RKA[st - 1], RKb[st] * dt, U, R};
+    if (want_lambda && c->fused && st == nStages - 1) {
+      stage.lamSum = c->lamSum;
+      stage.lamNan = c->lamNan;
+    }
     if (rhs(c, U, nullptr, stage, s, st > 0 || faces_known)) return 1;
   }
   return 0;
@@ -461,6 +472,7 @@ int fxg_set_state(fxg_ctx* c, const double* U) {
   if (!c || !U) return fail("NULL argument");
   c->faces_of = nullptr;
   c->rhs_stale = false;
+  c->lam_of = nullptr;
   HIPCHECK(hipMemcpyAsync(c->U, U, 5 * (size_t)c->np * c->K * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
@@ -526,8 +538,10 @@ int fxg_timestep_lserkw2(fxg_ctx* c, double t, double dt, int nStages, const dou
   (void)t;
   (void)RKc;  // stage times only enter time-dependent sources, which periodic flows do not have
   if (!c) return fail("ctx is NULL");
-  if (lserk(c, dt, nStages, RKA, RKb, c->U, c->R, c->stream, c->fused && c->faces_of == c->U)) return 1;
+  c->lam_of = nullptr;
+  if (lserk(c, dt, nStages, RKA, RKb, c->U, c->R, c->stream, c->fused && c->faces_of == c->U, true)) return 1;
   c->rhs_stale = c->fused;
+  if (c->fused) c->lam_of = c->U;
   c->last_b_dt = RKb[nStages - 1] * dt;
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
@@ -680,7 +694,10 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
 
 int fxg_calc_timestep(fxg_ctx* c, double CFL, double DFL, double* dt, int* errType, double* dtElem) {
   if (!c || !dt || !errType) return fail("NULL argument");
-  HIPCHECK(fxg::launch_calc_timestep(c->m, c->w, c->gas, c->U, CFL, DFL, c->stream, c->timer));
+  if (c->fused && c->lam_of == c->U)
+    HIPCHECK(fxg::launch_calc_timestep_from_sums(c->m, c->w, c->lamSum, c->lamNan, CFL, DFL, c->stream, c->timer));
+  else
+    HIPCHECK(fxg::launch_calc_timestep(c->m, c->w, c->gas, c->U, CFL, DFL, c->stream, c->timer));
   HIPCHECK(hipMemcpyAsync(c->dtHost, c->w.dtOut, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipMemcpyAsync(c->errHost, c->w.errOut, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if (dtElem) {

@@ -284,33 +284,52 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
     }
     __syncthreads();
   }
-  if (!act) return;
-  const double sjd = m.sJ[gid];
-  double ut[5];
+  // Ut = -Ut*sJ and the register update; with st.lamSum also CALCTIMESTEP's per-element spectral
+  // radii of the new state (the time-step kernel's arithmetic, dt_node_lambda)
+  double un[5] = {0, 0, 0, 0, 0};
+  double sjd = 0.0;
+  if (act) {
+    sjd = m.sJ[gid];
+    double ut[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) ut[v] = (-(double)acc[v]) * sjd;
-  const size_t o = 5 * gid;
-  if (st.mode == 0) {
+    for (int v = 0; v < 5; ++v) ut[v] = (-(double)acc[v]) * sjd;
+    const size_t o = 5 * gid;
+    if (st.mode == 0) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Ut[o + v] = ut[v];
-    return;
-  }
-  double un[5];
+      for (int v = 0; v < 5; ++v) Ut[o + v] = ut[v];
+    } else {
 #pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    const double r = (st.mode == 1) ? ut[v] : st.R[o + v] * st.minusA + ut[v];
-    st.R[o + v] = r;
-    un[v] = U[o + v] + r * st.b_dt;
-  }
+      for (int v = 0; v < 5; ++v) {
+        const double r = (st.mode == 1) ? ut[v] : st.R[o + v] * st.minusA + ut[v];
+        st.R[o + v] = r;
+        un[v] = U[o + v] + r * st.b_dt;
+      }
 #pragma unroll
-  for (int v = 0; v < 5; ++v) st.U[o + v] = un[v];
-  // face values of the new state for the next stage
+      for (int v = 0; v < 5; ++v) st.U[o + v] = un[v];
+      // face values of the new state for the next stage
 #pragma unroll 1
-  for (int ls = 1; ls <= 6; ++ls) {
-    if (!(sides & (1 << (ls - 1)))) continue;
-    double* dst = FUnext + 5 * (size_t)sIdx[face_point<N>(ls, i, j, k)];
+      for (int ls = 1; ls <= 6; ++ls) {
+        if (!(sides & (1 << (ls - 1)))) continue;
+        double* dst = FUnext + 5 * (size_t)sIdx[face_point<N>(ls, i, j, k)];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) dst[v] = un[v];
+        for (int v = 0; v < 5; ++v) dst[v] = un[v];
+      }
+    }
+  }
+  if (st.lamSum) {  // uniform over the launch; the direction loop's last barrier freed sX
+    double lam[6] = {0, 0, 0, 0, 0, 0};
+    int isnan_rho = 0;
+    if (act)
+      dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, m.MetricsAdv + 3 * gid,
+                     m.MetricsVisc + 3 * gid, lam, isnan_rho);
+    double mx[6];
+    int nanr = 0;
+    block_lambda_max<NT>(lam, isnan_rho, sX, reinterpret_cast<int*>(sX + 6 * (NT / 64)), t, mx, nanr);
+    if (t == 0) {
+      st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
+      st.lamSum[2 * e + 1] = mx[3] + mx[4] + mx[5];
+      st.lamNan[e] = nanr;
+    }
   }
 }
 

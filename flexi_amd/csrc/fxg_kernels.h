@@ -64,6 +64,10 @@ struct RKStage {
   double b_dt;     // RKb(iStage)*dt
   double* U;       // state, updated in place
   double* R;       // LSERK register Ut_tmp
+  // fused path, optional: per-element CALCTIMESTEP spectral-radius sums of the new state
+  // (2 per element: convective, viscous) and NaN-density flags
+  double* lamSum = nullptr;
+  int* lamNan = nullptr;
 };
 
 // Kernel ids for the optional event timer (fxg_profile_read).
@@ -123,6 +127,9 @@ hipError_t launch_scale(size_t n, double a, double* y, hipStream_t s);
 
 hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U,
                                 double CFL, double DFL, hipStream_t s, KernelTimer* timer);
+// CALCTIMESTEP from the spectral-radius sums a fused last stage stored for the current state.
+hipError_t launch_calc_timestep_from_sums(const DevMesh& m, const DevWork& w, const double* lamSum, const int* lamNan,
+                                          double CFL, double DFL, hipStream_t s, KernelTimer* timer);
 hipError_t launch_metrics_adv(const DevMesh& m, double KappasPr_max, double* adv, double* visc,
                               hipStream_t s);
 // Copies a (12, np, nElems) gradient array of type float/double into three FLEXI-layout

@@ -457,51 +457,20 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_dt_elem(const double* __restrict
                                                       double DFL, double* __restrict__ dtConv,
                                                       double* __restrict__ dtVisc, int* __restrict__ err) {
   constexpr int np = Dim<N>::np, NT = Dim<N>::NT;
-  __shared__ double red[6][NT / 64];
+  __shared__ double red[6 * (NT / 64)];
   __shared__ int nanflag[NT / 64];
   const int e = blockIdx.x, t = threadIdx.x;
   double lam[6] = {0, 0, 0, 0, 0, 0};
   int isnan_rho = 0;
   if (t < np) {
     const size_t gid = (size_t)np * e + t;
-    const double* c = U + 5 * gid;
-    const double srho = 1.0 / c[0];
-    const double vel[3] = {c[1] * srho, c[2] * srho, c[3] * srho};
-    const double p = gas.kappaM1 * (c[4] - 0.5 * (vel[0] * c[1] + vel[1] * c[2] + vel[2] * c[3]));
-    if (c[0] != c[0]) isnan_rho = 1;
-    const double cs = sqrt(gas.kappa * p * srho);
-    const double sj = m.sJ[gid];
-    const double vsJ[3] = {vel[0] * sj, vel[1] * sj, vel[2] * sj};
-    const double* M[3] = {m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      lam[d] = fabs(M[d][0] * vsJ[0] + M[d][1] * vsJ[1] + M[d][2] * vsJ[2]) + cs * m.MetricsAdv[3 * gid + d];
-      lam[3 + d] = gas.mu0 * srho * m.MetricsVisc[3 * gid + d];
-    }
+    dt_node_lambda(U + 5 * gid, gas, m.sJ[gid], m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, m.MetricsAdv + 3 * gid,
+                   m.MetricsVisc + 3 * gid, lam, isnan_rho);
   }
-  // wavefront max reduction; NaN propagates through the comparison chain as in MAX(..)
-  const int lane = t & 63, wv = t >> 6;
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    double x = lam[c];
-    for (int off = 32; off > 0; off >>= 1) {
-      const double y = __shfl_down(x, off, 64);
-      x = (y > x || y != y) ? y : x;
-    }
-    if (lane == 0) red[c][wv] = x;
-  }
-  const unsigned long long any = __ballot(isnan_rho);
-  if (lane == 0) nanflag[wv] = any ? 1 : 0;
-  __syncthreads();
+  double mx[6];
+  int nanr = 0;
+  block_lambda_max<NT>(lam, isnan_rho, red, nanflag, t, mx, nanr);
   if (t == 0) {
-    double mx[6];
-    int nanr = 0;
-    for (int c = 0; c < 6; ++c) mx[c] = red[c][0];
-    nanr = nanflag[0];
-    for (int w = 1; w < NT / 64; ++w) {
-      for (int c = 0; c < 6; ++c) mx[c] = (red[c][w] > mx[c] || red[c][w] != red[c][w]) ? red[c][w] : mx[c];
-      nanr |= nanflag[w];
-    }
     const double dc = CFL * 2. / (mx[0] + mx[1] + mx[2]);
     const double sv = mx[3] + mx[4] + mx[5];
     const double dv = (sv > 0.0) ? DFL * 4. / sv : 1.79769313486231571e308;
@@ -509,6 +478,21 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_dt_elem(const double* __restrict
     dtVisc[e] = dv;
     err[e] = nanr ? 1 : (dc != dc ? 2 : (dv != dv ? 3 : 0));
   }
+}
+
+// Per-element steps from the spectral-radius sums the fused last LSERK stage stored
+// (lamSum[2e]: convective, lamSum[2e+1]: viscous), as k_dt_elem finishes them.
+__global__ __launch_bounds__(256) void k_dt_finish(const double* __restrict__ lamSum, const int* __restrict__ lamNan,
+                                                 int K, double CFL, double DFL, double* __restrict__ dtConv,
+                                                 double* __restrict__ dtVisc, int* __restrict__ err) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= K) return;
+  const double dc = CFL * 2. / lamSum[2 * e];
+  const double sv = lamSum[2 * e + 1];
+  const double dv = (sv > 0.0) ? DFL * 4. / sv : 1.79769313486231571e308;
+  dtConv[e] = dc;
+  dtVisc[e] = dv;
+  err[e] = lamNan[e] ? 1 : (dc != dc ? 2 : (dv != dv ? 3 : 0));
 }
 
 // Minimum over elements of the convective and viscous steps and the error code.
@@ -665,6 +649,23 @@ static hipError_t dt_n(const DevMesh& m, const DevWork& w, const Gas& gas, const
   {
     TimedScope ts(tm, K_DT_ELEM, s);
     hipLaunchKernelGGL((k_dt_elem<N>), dim3(K), dim3(Dim<N>::NT), 0, s, U, m, gas, CFL, DFL, dtConv, dtVisc,
+                       w.errElem);
+  }
+  {
+    TimedScope ts(tm, K_DT_REDUCE, s);
+    hipLaunchKernelGGL(k_dt_reduce, dim3(1), dim3(1024), 0, s, dtConv, dtVisc, w.errElem, K, w.dtOut, w.errOut);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_calc_timestep_from_sums(const DevMesh& m, const DevWork& w, const double* lamSum, const int* lamNan,
+                                          double CFL, double DFL, hipStream_t s, KernelTimer* tm) {
+  const int K = m.nElems;
+  double* dtConv = w.dtElem;
+  double* dtVisc = w.dtElem + K;
+  {
+    TimedScope ts(tm, K_DT_ELEM, s);
+    hipLaunchKernelGGL(k_dt_finish, dim3((K + 255) / 256), dim3(256), 0, s, lamSum, lamNan, K, CFL, DFL, dtConv, dtVisc,
                        w.errElem);
   }
   {

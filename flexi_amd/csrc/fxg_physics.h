@@ -185,4 +185,56 @@ __device__ __forceinline__ void riemann_rot(const GasT<T>& g, int solver, bool s
   for (int k = 0; k < 5; ++k) F[k] = Fc[k] - T(0.5) * D[k];
 }
 
+// Per-node spectral radii of CALCTIMESTEP (src/equations/navierstokes/calctimestep.f90:121-190):
+// lam[d] convective = |Ja^d.v sJ| + c*MetricsAdv_d, lam[3+d] viscous = mu/rho*MetricsVisc_d. The
+// time-step kernel and the fused last LSERK stage both use this, so their time steps agree bitwise.
+__device__ __forceinline__ void dt_node_lambda(const double* c, const Gas& gas, double sj, const double* M0,
+                                               const double* M1, const double* M2, const double* adv,
+                                               const double* visc, double* lam, int& isnan_rho) {
+  const double srho = 1.0 / c[0];
+  const double vel[3] = {c[1] * srho, c[2] * srho, c[3] * srho};
+  const double p = gas.kappaM1 * (c[4] - 0.5 * (vel[0] * c[1] + vel[1] * c[2] + vel[2] * c[3]));
+  isnan_rho = (c[0] != c[0]) ? 1 : 0;
+  const double cs = sqrt(gas.kappa * p * srho);
+  const double vsJ[3] = {vel[0] * sj, vel[1] * sj, vel[2] * sj};
+  const double* M[3] = {M0, M1, M2};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lam[d] = fabs(M[d][0] * vsJ[0] + M[d][1] * vsJ[1] + M[d][2] * vsJ[2]) + cs * adv[d];
+    lam[3 + d] = gas.mu0 * srho * visc[d];
+  }
+}
+
+// Block-wide maxima of lam[6] (NaN propagates, as MAX(..) in the reference) and OR of the NaN-density
+// flags; red needs 6*(NT/64) doubles and nanflag NT/64 ints of LDS. Every thread must call it; the
+// result is valid in thread 0.
+template <int NT>
+__device__ __forceinline__ void block_lambda_max(double* lam, int isnan_rho, double* red, int* nanflag, int t,
+                                                 double* mx, int& nanr) {
+  const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double x = lam[c];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double y = __shfl_down(x, off, 64);
+      x = (y > x || y != y) ? y : x;
+    }
+    if (lane == 0) red[c * (NT / 64) + wv] = x;
+  }
+  const unsigned long long any = __ballot(isnan_rho);
+  if (lane == 0) nanflag[wv] = any ? 1 : 0;
+  __syncthreads();
+  if (t == 0) {
+    for (int c = 0; c < 6; ++c) mx[c] = red[c * (NT / 64)];
+    nanr = nanflag[0];
+    for (int w = 1; w < NT / 64; ++w) {
+      for (int c = 0; c < 6; ++c) {
+        const double r = red[c * (NT / 64) + w];
+        mx[c] = (r > mx[c] || r != r) ? r : mx[c];
+      }
+      nanr |= nanflag[w];
+    }
+  }
+}
+
 }  // namespace fxg

@@ -214,3 +214,20 @@ def test_fused_steps_reuse_face_values_between_calls():
     U2o = np.asfortranarray(U2.copy())
     oracle.step(U2o, 0.0, dt, orc.F(U.shape))
     assert rel(gpu.get_state() - U2, U2o - U2) <= 1e-11
+
+
+def test_fused_last_stage_time_step_equals_time_step_kernel():
+    # after a fused step, CALCTIMESTEP uses the spectral radii the last stage stored; a fresh
+    # set_state forces the time-step kernel on the same state
+    mesh, m, oracle, gpu, U = setup(7, orc.NODE_GL, True, orc.RIEMANN_ROE, shape=(3, 2, 2))
+    gpu.set_state(U)
+    for it in range(2):
+        dt, err = gpu.CalcTimeStep()
+        gpu.dt = dt
+        gpu.TimeStepByLSERKW2(0.0)
+    dt1, err1, e1 = gpu.CalcTimeStep(with_elements=True)
+    gpu.set_state(gpu.get_state())
+    dt2, err2, e2 = gpu.CalcTimeStep(with_elements=True)
+    assert err1 == err2 == 0
+    assert abs(dt1 - dt2) <= 1e-15 * dt2
+    np.testing.assert_allclose(e1, e2, rtol=1e-15, atol=0)
