@@ -273,7 +273,7 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   if (c->fused) {
     char* FG;
     if (alloc_owned(c, &c->fw.FU[0], 10 * face) || alloc_owned(c, &c->fw.FU[1], 10 * face) ||
-        alloc_owned(c, &FG, 8 * face * gsz) || alloc_owned(c, &c->fw.Jsoa, 10 * vol) ||
+        alloc_owned(c, &FG, 8 * face * gsz) || alloc_owned(c, &c->fw.Jsoa, fxg::packed_metrics_size(c->N, K)) ||
         alloc_owned(c, &c->lamSum, 2 * (size_t)K) || alloc_owned(c, &c->lamNan, (size_t)K))
       return 1;
     c->fw.FG = FG;

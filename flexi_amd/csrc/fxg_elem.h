@@ -184,4 +184,20 @@ __device__ __forceinline__ double face_frame(const DevMesh& m, int e, int ls, in
   return se;
 }
 
+// Boundary nodes of an element (on at least one face): NB of them, numbered plane by plane in
+// volume order (k = 0 plane: all; 0 < k < N: the ring j = 0, i = 0 | N, j = N; k = N: all), so
+// that consecutive boundary lanes address consecutive entries.
+template <int N>
+struct Bnd {
+  static constexpr int n = N + 1, nf = n * n, NB = 2 * nf + (N - 1) * 4 * N;
+};
+template <int N>
+__device__ __forceinline__ int bnd_rank(int i, int j, int k) {
+  constexpr int n = N + 1, nf = n * n;
+  if (k == 0) return i + n * j;
+  if (k == N) return nf + (N - 1) * 4 * N + i + n * j;
+  const int ring = (j == 0) ? i : (j == N ? n + 2 * (N - 1) + i : n + 2 * (j - 1) + (i == 0 ? 0 : 1));
+  return nf + (k - 1) * 4 * N + ring;
+}
+
 }  // namespace fxg

@@ -63,9 +63,16 @@ __global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restric
   const int sides = node_sides<N>(i, j, k);
   if (!sides) return;
   const GasT<RF> gt(gas);
-  const double* Je = Jsoa + (size_t)10 * np * e;
+  // metric terms of the boundary nodes only, rows of NB per element (Jsoa layout)
+  constexpr int NB = Bnd<N>::NB;
+  const double* Je = Jsoa + (size_t)10 * NB * e;
   RF Ja[9], sj;
-  load_metrics<N, RF>(Je, t, Ja, sj);
+  {
+    const int b = bnd_rank<N>(i, j, k);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) Ja[r] = (RF)Je[r * NB + b];
+    sj = (RF)Je[9 * NB + b];
+  }
   RF g[12];
   lift_node<N, RF>(t, sP, sB, sNb, Ja, sj, g);
   RF fx[4], fy[4], fz[4];
@@ -270,16 +277,18 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
         const RF rs = rho + (RF)sA[id];
         const RF us = uu + (RF)sA[3 * np + id], vs = vv + (RF)sA[4 * np + id], ws = ww + (RF)sA[5 * np + id];
         const RF ps = pp + (RF)sA[np + id], Hs = HH + (RF)sA[2 * np + id];
-        const RF a0 = RF(0.5) * (m0 + (RF)sM[id]), a1 = RF(0.5) * (m1 + (RF)sM[np + id]),
-                 a2 = RF(0.5) * (m2 + (RF)sM[2 * np + id]);
-        const RF mv = a0 * us + a1 * vs + a2 * ws;
-        const RF qq = RF(0.25) * rs * mv;
-        const RF dv = (RF)DV[l + n * me];
-        acc[0] += dv * (RF(0.5) * rs * mv);
-        acc[1] += dv * (qq * us + a0 * ps);
-        acc[2] += dv * (qq * vs + a1 * ps);
-        acc[3] += dv * (qq * ws + a2 * ps);
-        acc[4] += dv * (qq * Hs);
+        // Pirozzoli two-point flux with the metric average a = b/2 (SplitVolumeFluxPI,
+        // splitflux.f90:603-666), evaluated as 2F with b = Ja_i + Ja_l and weighted by DV/2: the
+        // powers of two make every product and sum round exactly as in F and DV*F
+        const RF b0 = m0 + (RF)sM[id], b1 = m1 + (RF)sM[np + id], b2 = m2 + (RF)sM[2 * np + id];
+        const RF mb = b0 * us + b1 * vs + b2 * ws;  // 2 (a . u_sum)
+        const RF Q = RF(0.25) * rs * mb;           // 2 qq = rs (a . u_sum) / 2
+        const RF dv = (RF)DV[l + n * me], dvh = RF(0.5) * dv;
+        acc[0] += dv * Q;
+        acc[1] += dvh * (Q * us + b0 * ps);
+        acc[2] += dvh * (Q * vs + b1 * ps);
+        acc[3] += dvh * (Q * ws + b2 * ps);
+        acc[4] += dvh * (Q * Hs);
       }
     }
     __syncthreads();
@@ -395,18 +404,25 @@ hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int rie
 }
 
 // Packs the FLEXI-layout metric terms (3,np,K) x 3 and sJ (np,K) into rows of np per element.
-__global__ __launch_bounds__(256) void k_pack_metrics(DevMesh m, int np, size_t nNodes, double* __restrict__ J) {
+// Packs the metric terms of the boundary nodes, (3,np,K) x 3 and sJ (np,K), into rows of NB
+// per element in boundary-node order (bnd_rank).
+template <int N>
+__global__ __launch_bounds__(256) void k_pack_metrics(DevMesh m, size_t nNodes, double* __restrict__ J) {
+  constexpr int n = N + 1, np = D<N>::np, NB = Bnd<N>::NB;
   const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nNodes) return;
-  const size_t e = x / np, t = x - e * np;
-  double* Je = J + 10 * np * e + t;
+  const size_t e = x / np;
+  const int t = (int)(x - e * np);
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  if (!node_sides<N>(i, j, k)) return;
+  double* Je = J + 10 * NB * e + bnd_rank<N>(i, j, k);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    Je[(0 + c) * np] = m.Mf[3 * x + c];
-    Je[(3 + c) * np] = m.Mg[3 * x + c];
-    Je[(6 + c) * np] = m.Mh[3 * x + c];
+    Je[(0 + c) * NB] = m.Mf[3 * x + c];
+    Je[(3 + c) * NB] = m.Mg[3 * x + c];
+    Je[(6 + c) * NB] = m.Mh[3 * x + c];
   }
-  Je[9 * np] = m.sJ[x];
+  Je[9 * NB] = m.sJ[x];
 }
 
 }  // namespace
@@ -422,10 +438,22 @@ hipError_t launch_faces_of(const DevMesh& m, const double* U, double* FU, hipStr
   return hipGetLastError();
 }
 
+size_t packed_metrics_size(int N, int nElems) {
+  const int n = N + 1;
+  return (size_t)10 * (2 * n * n + (N - 1) * 4 * N) * nElems;
+}
+
 hipError_t launch_pack_metrics(const DevMesh& m, double* Jsoa, hipStream_t s) {
   const int np = (m.N + 1) * (m.N + 1) * (m.N + 1);
   const size_t nNodes = (size_t)np * m.nElems;
-  hipLaunchKernelGGL(k_pack_metrics, dim3((unsigned)((nNodes + 255) / 256)), dim3(256), 0, s, m, np, nNodes, Jsoa);
+  const dim3 g((unsigned)((nNodes + 255) / 256)), b(256);
+  switch (m.N) {
+#define FXG_CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_pack_metrics<NN>), g, b, 0, s, m, nNodes, Jsoa); break;
+    FXG_CASE(2) FXG_CASE(3) FXG_CASE(4) FXG_CASE(5) FXG_CASE(6) FXG_CASE(7)
+#undef FXG_CASE
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

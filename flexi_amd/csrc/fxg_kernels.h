@@ -93,11 +93,13 @@ hipError_t launch_rhs(const DevMesh& m, const DevWork& w, const Gas& gas, int no
 struct FusedWork {
   double* FU[2];  // face states, 5 per side point and slot, double-buffered across stages
   void* FG;       // viscous normal face fluxes f_v.(n SurfElem), 4 per side point and slot
-  double* Jsoa;   // metric terms per element as rows of (N+1)^3: f x,y,z, g x,y,z, h x,y,z, sJ
+  double* Jsoa;   // metric terms of the boundary nodes, rows of NB per element: Ja^1..3 x,y,z, sJ
 };
+// Doubles of FusedWork::Jsoa.
+size_t packed_metrics_size(int N, int nElems);
 // Face states of U into a face array laid out like FusedWork::FU (Gauss-Lobatto).
 hipError_t launch_faces_of(const DevMesh& m, const double* U, double* FU, hipStream_t s);
-// Fills FusedWork::Jsoa from the FLEXI-layout metric arrays of `m`.
+// Fills FusedWork::Jsoa (boundary-node metric terms) from the FLEXI-layout metric arrays of `m`.
 hipError_t launch_pack_metrics(const DevMesh& m, double* Jsoa, hipStream_t s);
 bool fused_supported(int N, int nodeType, int split, long long nSides);
 // One stage of the fused path. FU[cur] must hold the face states of U unless faces_fresh is
