@@ -161,11 +161,12 @@ def adjoint_ratio():
     ubar = torch.ones_like(U0d)
     t_bwd = timed(lambda: s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts, every, ckpt.data_ptr()))
     state = U0.nbytes
+    face_states = 10 * (5 + 1) ** 2 * mesh.nSides * 8  # both slots of every side, 5 doubles per point
     s.close()
     return {"ratio": t_bwd / t_fwd, "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "adjoint_s": t_bwd,
             "gradient_total_ratio": (t_fwd_ck + t_bwd) / t_fwd,
             "memory_bytes": {"checkpoints": ckpt.numel() * 8, "segment_stage_states": every * 5 * state,
-                             "work": 3 * state},
+                             "segment_face_states": every * 5 * face_states, "work": 3 * state},
             "case": "TGV 16^3, N=5 GL split PI Roe, 200 steps, checkpoints every 20 steps"}
 
 

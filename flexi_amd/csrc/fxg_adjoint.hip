@@ -682,25 +682,25 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_stage_update(const double* __r
   }
 }
 
-// y += a*x and y = a*y over n doubles (the LSERK adjoint's register updates)
+// y += a*x over n doubles
 __global__ __launch_bounds__(256) void k_axpy(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] += a * x[i];
 }
-__global__ __launch_bounds__(256) void k_scale(size_t n, double a, double* __restrict__ y) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] *= a;
-}
 
 // The VJP at U of lam (+ cub*ub when ub is given). stage == nullptr: out = result. Otherwise a
 // reverse LSERK stage: stage[0] = a, and ub += result, rbar(=lam) = a*(lam + cub*ub) in place.
+// faces_ready: w.FUa already holds the face states of U.
 template <int N>
 hipError_t vjp_n(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                 const double* lam, const double* ub, double cub, double* out, hipStream_t s, const double* stage) {
+                 const double* lam, const double* ub, double cub, double* out, hipStream_t s, const double* stage,
+                 bool faces_ready) {
   const int K = m.nElems;
   constexpr int NT = D<N>::NT;
-  hipError_t err = launch_faces_of(m, U, w.FUa, s);
-  if (err != hipSuccess) return err;
+  if (!faces_ready) {
+    const hipError_t err = launch_faces_of(m, U, w.FUa, s);
+    if (err != hipSuccess) return err;
+  }
   if (riemann == 1)
     hipLaunchKernelGGL((k_adj_euler<N, 1>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, gas, out, w.XU, w.Xg);
   else if (riemann == 3)
@@ -722,14 +722,14 @@ hipError_t vjp_n(const DevMesh& m, const AdjointWork& w, const Gas& gas, int rie
 namespace {
 hipError_t vjp_dispatch(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
                         const double* lam, const double* ub, double cub, double* out, hipStream_t s,
-                        const double* stage) {
+                        const double* stage, bool faces_ready) {
   switch (m.N) {
-    case 2: return vjp_n<2>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
-    case 3: return vjp_n<3>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
-    case 4: return vjp_n<4>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
-    case 5: return vjp_n<5>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
-    case 6: return vjp_n<6>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
-    case 7: return vjp_n<7>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage);
+    case 2: return vjp_n<2>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
+    case 3: return vjp_n<3>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
+    case 4: return vjp_n<4>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
+    case 5: return vjp_n<5>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
+    case 6: return vjp_n<6>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
+    case 7: return vjp_n<7>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
     default: return hipErrorInvalidValue;
   }
 }
@@ -737,22 +737,21 @@ hipError_t vjp_dispatch(const DevMesh& m, const AdjointWork& w, const Gas& gas, 
 
 hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
                           const double* lam, double* out, hipStream_t s) {
-  return vjp_dispatch(m, w, gas, riemann, U, lam, nullptr, 0.0, out, s, nullptr);
+  return vjp_dispatch(m, w, gas, riemann, U, lam, nullptr, 0.0, out, s, nullptr, false);
 }
 
 hipError_t launch_adjoint_stage(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                                double* rbar, double* ubar, double cub, double a, double* tmp, hipStream_t s) {
+                                const double* FU, double* rbar, double* ubar, double cub, double a, double* tmp,
+                                hipStream_t s) {
   const double stage[1] = {a};
-  return vjp_dispatch(m, w, gas, riemann, U, rbar, ubar, cub, tmp, s, stage);
+  if (!FU) return vjp_dispatch(m, w, gas, riemann, U, rbar, ubar, cub, tmp, s, stage, false);
+  AdjointWork wf = w;
+  wf.FUa = const_cast<double*>(FU);  // read only by the adjoint kernels
+  return vjp_dispatch(m, wf, gas, riemann, U, rbar, ubar, cub, tmp, s, stage, true);
 }
 
 hipError_t launch_axpy(size_t n, double a, const double* x, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, x, y);
-  return hipGetLastError();
-}
-
-hipError_t launch_scale(size_t n, double a, double* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, y);
   return hipGetLastError();
 }
 

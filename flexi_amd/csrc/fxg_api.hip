@@ -646,15 +646,19 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
   if (nSteps == 0) return 0;
   const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const size_t vol5 = (size_t)5 * c->np * c->K, bytes = vol5 * sizeof(double);
+  const size_t face10 = (size_t)10 * c->nf * c->S;  // both slots of every side, 5 per point
   const int every = checkpointEvery < nSteps ? checkpointEvery : nSteps;
   const int nck = (nSteps + every - 1) / every;
   DevBuffers buf;
   const double* ck = ckptIn;
+  // the stage states of one segment and their face states, written by the recomputation itself
   double* seg = buf.get((size_t)every * nStages * vol5);
+  double* segF = buf.get((size_t)every * nStages * face10);
   double* Uw = buf.get(vol5);
   double* R = buf.get(vol5);
   double* rbar = buf.get(vol5);
-  if (!seg || !Uw || !R || !rbar) return fail("fxg_adjoint_n_steps: out of device memory for the stage states");
+  if (!seg || !segF || !Uw || !R || !rbar)
+    return fail("fxg_adjoint_n_steps: out of device memory for the stage states");
   if (!ck) {
     // forward sweep with a checkpoint of U every `every` steps
     double* ckw = buf.get((size_t)nck * vol5);
@@ -663,7 +667,7 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
     for (int it = 0; it < nSteps; ++it) {
       if (it % every == 0)
         HIPCHECK(hipMemcpyAsync(ckw + (size_t)(it / every) * vol5, Uw, bytes, hipMemcpyDeviceToDevice, s));
-      if (lserk(c, dt[it], nStages, RKA, RKb, Uw, R, s)) return 1;
+      if (lserk(c, dt[it], nStages, RKA, RKb, Uw, R, s, it > 0)) return 1;
     }
     ck = ckw;
   }
@@ -674,18 +678,28 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
   HIPCHECK(hipMemsetAsync(rbar, 0, bytes, s));
   for (int sg = nck - 1; sg >= 0; --sg) {
     const int s0 = sg * every, s1 = (s0 + every < nSteps) ? s0 + every : nSteps;
-    HIPCHECK(hipMemcpyAsync(Uw, ck + (size_t)sg * vol5, bytes, hipMemcpyDeviceToDevice, s));
-    for (int it = s0; it < s1; ++it)
-      for (int st = 0; st < nStages; ++st) {
-        HIPCHECK(hipMemcpyAsync(seg + ((size_t)(it - s0) * nStages + st) * vol5, Uw, bytes, hipMemcpyDeviceToDevice, s));
-        if (lserk_stage(c, dt[it], st, RKA, RKb, Uw, R, s)) return 1;
-      }
-    for (int it = s1 - 1; it >= s0; --it)
-      for (int st = nStages - 1; st >= 0; --st) {
-        const double* Us = seg + ((size_t)(it - s0) * nStages + st) * vol5;
-        HIPCHECK(fxg::launch_adjoint_stage(c->m, c->aw, c->gas, c->p.riemann, Us, rbar, ubar, RKb[st] * dt[it],
-                                           st == 0 ? 0.0 : -RKA[st - 1], c->adjTmp, s));
-      }
+    const int nst = (s1 - s0) * nStages;
+    HIPCHECK(hipMemcpyAsync(seg, ck + (size_t)sg * vol5, bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(fxg::launch_faces_of(c->m, seg, segF, s));
+    // stage x reads seg[x] and its faces segF[x] and writes the next stage state and its faces
+    // to seg[x+1], segF[x+1] (the end state of the segment to scratch)
+    for (int x = 0; x < nst; ++x) {
+      const int it = s0 + x / nStages, st = x % nStages;
+      const bool last = x + 1 == nst;
+      fxg::FusedWork fw = c->fw;
+      fw.FU[0] = segF + (size_t)x * face10;
+      fw.FU[1] = last ? c->fw.FU[0] : segF + (size_t)(x + 1) * face10;
+      const fxg::RKStage stage{st == 0 ? 1 : 2, st == 0 ? 0.0 : -RKA[st - 1], RKb[st] * dt[it],
+                               last ? Uw : seg + (size_t)(x + 1) * vol5, R};
+      HIPCHECK(fxg::launch_rhs_fused(c->m, fw, c->gas, c->p.riemann, 0, seg + (size_t)x * vol5, nullptr, stage, true,
+                                     0, nullptr, s, c->timer));
+    }
+    for (int x = nst - 1; x >= 0; --x) {
+      const int it = s0 + x / nStages, st = x % nStages;
+      HIPCHECK(fxg::launch_adjoint_stage(c->m, c->aw, c->gas, c->p.riemann, seg + (size_t)x * vol5,
+                                         segF + (size_t)x * face10, rbar, ubar, RKb[st] * dt[it],
+                                         st == 0 ? 0.0 : -RKA[st - 1], c->adjTmp, s));
+    }
   }
   HIPCHECK(hipStreamSynchronize(s));
   c->faces_of = nullptr;

@@ -121,11 +121,13 @@ struct AdjointWork {
 hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
                           const double* lam, double* out, hipStream_t s);
 // One reverse LSERK stage at the stage state U: lam = rbar + cub*ubar; ubar += (dUt/dU)^T lam;
-// rbar = a*lam. `tmp` is a state-sized scratch array.
+// rbar = a*lam. `FU`: the face states of U in the fused path's face layout (nullptr: computed
+// into w.FUa). `tmp` is a state-sized scratch array.
 hipError_t launch_adjoint_stage(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                                double* rbar, double* ubar, double cub, double a, double* tmp, hipStream_t s);
+                                const double* FU, double* rbar, double* ubar, double cub, double a, double* tmp,
+                                hipStream_t s);
+// y += a*x over n doubles
 hipError_t launch_axpy(size_t n, double a, const double* x, double* y, hipStream_t s);
-hipError_t launch_scale(size_t n, double a, double* y, hipStream_t s);
 
 hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& gas, const double* U,
                                 double CFL, double DFL, hipStream_t s, KernelTimer* timer);
