@@ -297,12 +297,15 @@ __device__ __forceinline__ void rot_state_adjoint(const double* c, const Gas& g,
 }
 
 // ---------------------------------------------------------------------------------------------
+// Roe without entropy fix at N=5,6 (256/384-lane workgroups of 216/343 active lanes): three waves
+// per SIMD at 168 VGPRs with a 20-28 byte spill instead of two at 174-176. (At N=7 a 512-lane
+// workgroup cannot gain a wave that way; at N=3,4 the spill is larger.)
 template <int N, int SOLVER>
-__global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict__ U, DevMesh m,
-                                                      const double* __restrict__ FUa, const double* __restrict__ lam,
-                                                      const double* __restrict__ ub, double cub,
-                                                      Gas gas, double* __restrict__ out, double* __restrict__ XU,
-                                                      double* __restrict__ Xg) {
+__global__ __launch_bounds__(D<N>::NT)
+__attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) void k_adj_euler(
+    const double* __restrict__ U, DevMesh m, const double* __restrict__ FUa, double* __restrict__ lam,
+    double* __restrict__ ub, double cub, Gas gas, double* __restrict__ out, double* __restrict__ XU,
+    double* __restrict__ Xg, Pending pd) {
   constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
   __shared__ double sA[6 * np];   // rho, p, H, u, v, w
   __shared__ double sMu[5 * np];  // -sJ*lam
@@ -314,16 +317,46 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict
   const int e = blockIdx.x, t = threadIdx.x;
   const bool act = t < np;
   const size_t gid = (size_t)np * e + (act ? t : 0);
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
   for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
+  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot<N>(m, e, x);
+  if (pd.XU) __syncthreads();  // uniform over the launch
   if (act) {
     const Prim q = cons_to_prim(U + 5 * gid, gas);
     sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (U[5 * gid + 4] + q.p) / q.rho;
     sA[3 * np + t] = q.u; sA[4 * np + t] = q.v; sA[5 * np + t] = q.w;
     const double sj = m.sJ[gid];
+    double l5[5];
+    if (pd.XU) {
+      // the register update of the previous reverse stage: ub += its out + the neighbours' face
+      // parts, rbar = a*(rbar + cub*ub) (rbar lives in lam)
+      double g5[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) sMu[v * np + t] = -sj * (lam[5 * gid + v] + (ub ? cub * ub[5 * gid + v] : 0.0));
+      for (int v = 0; v < 5; ++v) g5[v] = out[5 * gid + v];
+      const int sides = node_sides<N>(i, j, k);
+#pragma unroll 1
+      for (int ls = 1; ls <= 6; ++ls) {
+        if (!(sides & (1 << (ls - 1)))) continue;
+        const unsigned nb = slot_nbr<N>(sIdx[face_point<N>(ls, i, j, k)]);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) g5[v] += pd.XU[5 * (size_t)nb + v];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double u0 = ub[5 * gid + v];
+        const double r = pd.a * (lam[5 * gid + v] + pd.cub * u0);
+        const double u1 = u0 + g5[v];
+        lam[5 * gid + v] = r;
+        ub[5 * gid + v] = u1;
+        l5[v] = r + cub * u1;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) l5[v] = lam[5 * gid + v] + (ub ? cub * ub[5 * gid + v] : 0.0);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sMu[v * np + t] = -sj * l5[v];
   }
-  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot<N>(m, e, x);
   __syncthreads();
 
   // own face points: the neighbour's share of the viscous face term, and the Riemann adjoint
@@ -394,7 +427,6 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict
 
   // split-flux adjoint: each pair (i,l) contributes J_s^T (DV(l,i) mu_i + DV(i,l) mu_l) to both
   // nodes' primitive adjoints, J_s the flux Jacobian w.r.t. the pair sums (rho,u,v,w,p,H)
-  const int i = t % n, j = (t / n) % n, k = t / (n * n);
   double lrho = 0, lu = 0, lv = 0, lw = 0, lp = 0, lH = 0;
   double rho = 0, uu = 0, vv = 0, ww = 0, pp = 0, HH = 0, mu[5] = {0, 0, 0, 0, 0};
   if (act) {
@@ -458,15 +490,17 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_euler(const double* __restrict
 }
 
 // ---------------------------------------------------------------------------------------------
+// XU: the Riemann part of the neighbour's face adjoint (k_adj_euler); the lifting part w.r.t. the
+// neighbour's (u,v,w,T), taken to its conservative variables with its face state, is added here.
 template <int N>
 __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict__ U, DevMesh m,
                                                      const double* __restrict__ FUa, const double* __restrict__ lam,
                                                      const double* __restrict__ ub, double cub,
                                                      const double* __restrict__ Xg, Gas gas, double* __restrict__ out,
-                                                     double* __restrict__ Xq) {
+                                                     double* __restrict__ XU) {
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
   __shared__ double sP[4 * np];    // u, v, w, T
-  __shared__ double sNb[4 * NF];   // neighbour u, v, w, T per own face point
+  __shared__ double sNb[5 * NF];   // neighbour u, v, w, T, rho per own face point
   __shared__ double sMu[4 * np];   // -sJ*lam, momentum and energy
   __shared__ double sK[12 * np];   // lifting adjoint per reference derivative: rows 4*d + var
   __shared__ unsigned sIdx[NF];
@@ -489,7 +523,7 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict_
     const unsigned own = face_slot<N>(m, e, x);
     sIdx[x] = own;
     const Prim qn = cons_to_prim(FUa + 5 * (size_t)slot_nbr<N>(own), gas);
-    sNb[x] = qn.u; sNb[NF + x] = qn.v; sNb[2 * NF + x] = qn.w; sNb[3 * NF + x] = qn.T;
+    sNb[x] = qn.u; sNb[NF + x] = qn.v; sNb[2 * NF + x] = qn.w; sNb[3 * NF + x] = qn.T; sNb[4 * NF + x] = qn.rho;
   }
   __syncthreads();
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
@@ -567,20 +601,28 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict_
 #pragma unroll
       for (int v2 = 0; v2 < 4; ++v2)
         sK[(4 * d + v2) * np + t] = Ja[3 * d] * nu[v2] + Ja[3 * d + 1] * nu[4 + v2] + Ja[3 * d + 2] * nu[8 + v2];
-    // face corrections 0.5*L_hat*(q_nb - q_own).(n SurfElem): own part here, neighbour's via Xq
+    // face corrections 0.5*L_hat*(q_nb - q_own).(n SurfElem): own part here, the neighbour's
+    // part to its conservative variables (its state is the face state FUa of the neighbour slot)
+    // and into its exchange slot
     const double lhm = sB[B.L_HatMinus()], lhp = sB[B.L_HatPlus() + N];
 #pragma unroll 1
     for (int ls = 1; ls <= 6; ++ls) {
       if (!(sides & (1 << (ls - 1)))) continue;
       const int dir = dir_of(ls);
       const double s = normal_sign(ls) * 0.5 * (minus_side(ls) ? lhm : lhp);
-      const unsigned own = sIdx[face_point<N>(ls, i, j, k)];
+      const int x = face_point<N>(ls, i, j, k);
+      const unsigned own = sIdx[x];
+      double z[4];
 #pragma unroll
       for (int v2 = 0; v2 < 4; ++v2) {
-        const double z = s * (Ja[3 * dir] * nu[v2] + Ja[3 * dir + 1] * nu[4 + v2] + Ja[3 * dir + 2] * nu[8 + v2]);
-        lq[v2] -= z;
-        Xq[4 * (size_t)own + v2] = z;
+        z[v2] = s * (Ja[3 * dir] * nu[v2] + Ja[3 * dir + 1] * nu[4 + v2] + Ja[3 * dir + 2] * nu[8 + v2]);
+        lq[v2] -= z[v2];
       }
+      const Prim qn{sNb[4 * NF + x], sNb[x], sNb[NF + x], sNb[2 * NF + x], 0.0, sNb[3 * NF + x]};
+      double lP[5];
+      prim_adjoint(qn, 0.0, gas, 0.0, z[0], z[1], z[2], 0.0, z[3], 0.0, lP);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) XU[5 * (size_t)own + v] += lP[v];
     }
   }
   __syncthreads();
@@ -605,10 +647,10 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict_
 }
 
 // ---------------------------------------------------------------------------------------------
+// out += the neighbours' face parts (XU of the neighbour slot of every own face point)
 template <int N>
-__global__ __launch_bounds__(D<N>::NT) void k_adj_gather(const double* __restrict__ U, DevMesh m,
-                                                       const double* __restrict__ XU, const double* __restrict__ Xq,
-                                                       Gas gas, double* __restrict__ out) {
+__global__ __launch_bounds__(D<N>::NT) void k_adj_gather(DevMesh m, const double* __restrict__ XU,
+                                                       double* __restrict__ out) {
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
   __shared__ unsigned sIdx[NF];
   const int e = blockIdx.x, t = threadIdx.x;
@@ -619,30 +661,22 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_gather(const double* __restric
   const int sides = node_sides<N>(i, j, k);
   if (!sides) return;
   const size_t gid = (size_t)np * e + t;
-  const Prim q = cons_to_prim(U + 5 * gid, gas);
-  double lU[5] = {0, 0, 0, 0, 0}, lq[4] = {0, 0, 0, 0};
+  double lU[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
   for (int ls = 1; ls <= 6; ++ls) {
     if (!(sides & (1 << (ls - 1)))) continue;
     const unsigned nb = slot_nbr<N>(sIdx[face_point<N>(ls, i, j, k)]);
 #pragma unroll
     for (int v = 0; v < 5; ++v) lU[v] += XU[5 * (size_t)nb + v];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) lq[v] += Xq[4 * (size_t)nb + v];
   }
-  double lP[5];
-  prim_adjoint(q, U[5 * gid + 4], gas, 0.0, lq[0], lq[1], lq[2], 0.0, lq[3], 0.0, lP);
 #pragma unroll
-  for (int v = 0; v < 5; ++v) out[5 * gid + v] += lU[v] + lP[v];
+  for (int v = 0; v < 5; ++v) out[5 * gid + v] += lU[v];
 }
 
-// One reverse LSERK stage's register updates, with the neighbours' face parts of the adjoint:
-// lam = rbar + cub*ubar (the adjoint the stage's right-hand side saw), then
-// ubar += out + face parts,  rbar = a*lam  (a = -RKA of the stage, 0 for the first).
+// The register update of a reverse LSERK stage (the last one of a chain; the others are applied
+// by the next stage's k_adj_euler): ub += out + the neighbours' face parts, rbar = a*(rbar + cub*ub).
 template <int N>
-__global__ __launch_bounds__(D<N>::NT) void k_adj_stage_update(const double* __restrict__ U, DevMesh m,
-                                                             const double* __restrict__ XU,
-                                                             const double* __restrict__ Xq, Gas gas,
+__global__ __launch_bounds__(D<N>::NT) void k_adj_stage_update(DevMesh m, const double* __restrict__ XU,
                                                              const double* __restrict__ out, double* __restrict__ ub,
                                                              double* __restrict__ rbar, double cub, double a) {
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
@@ -654,31 +688,21 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_stage_update(const double* __r
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
   const int sides = node_sides<N>(i, j, k);
   const size_t gid = (size_t)np * e + t;
-  double lU[5];
+  double g5[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) lU[v] = out[5 * gid + v];
-  if (sides) {
-    const Prim q = cons_to_prim(U + 5 * gid, gas);
-    double lq[4] = {0, 0, 0, 0};
+  for (int v = 0; v < 5; ++v) g5[v] = out[5 * gid + v];
 #pragma unroll 1
-    for (int ls = 1; ls <= 6; ++ls) {
-      if (!(sides & (1 << (ls - 1)))) continue;
-      const unsigned nb = slot_nbr<N>(sIdx[face_point<N>(ls, i, j, k)]);
+  for (int ls = 1; ls <= 6; ++ls) {
+    if (!(sides & (1 << (ls - 1)))) continue;
+    const unsigned nb = slot_nbr<N>(sIdx[face_point<N>(ls, i, j, k)]);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) lU[v] += XU[5 * (size_t)nb + v];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) lq[v] += Xq[4 * (size_t)nb + v];
-    }
-    double lP[5];
-    prim_adjoint(q, U[5 * gid + 4], gas, 0.0, lq[0], lq[1], lq[2], 0.0, lq[3], 0.0, lP);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) lU[v] += lP[v];
+    for (int v = 0; v < 5; ++v) g5[v] += XU[5 * (size_t)nb + v];
   }
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const double u0 = ub[5 * gid + v];
     rbar[5 * gid + v] = a * (rbar[5 * gid + v] + cub * u0);
-    ub[5 * gid + v] = u0 + lU[v];
+    ub[5 * gid + v] = u0 + g5[v];
   }
 }
 
@@ -688,67 +712,96 @@ __global__ __launch_bounds__(256) void k_axpy(size_t n, double a, const double* 
   if (i < n) y[i] += a * x[i];
 }
 
-// The VJP at U of lam (+ cub*ub when ub is given). stage == nullptr: out = result. Otherwise a
-// reverse LSERK stage: stage[0] = a, and ub += result, rbar(=lam) = a*(lam + cub*ub) in place.
-// faces_ready: w.FUa already holds the face states of U.
+// The transposed-Jacobian parts of one right-hand side at U (face states FU): out = the element's
+// own parts, XU = the parts of the neighbours' face nodes. lam (+ cub*ub when ub is given) is the
+// adjoint of Ut; with pd.XU a previous stage's register update is applied first (k_adj_euler).
 template <int N>
-hipError_t vjp_n(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                 const double* lam, const double* ub, double cub, double* out, hipStream_t s, const double* stage,
-                 bool faces_ready) {
+hipError_t adj_parts(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
+                     const double* FU, double* lam, double* ub, double cub, double* out, double* XU, Pending pd,
+                     hipStream_t s) {
   const int K = m.nElems;
   constexpr int NT = D<N>::NT;
-  if (!faces_ready) {
-    const hipError_t err = launch_faces_of(m, U, w.FUa, s);
-    if (err != hipSuccess) return err;
-  }
   if (riemann == 1)
-    hipLaunchKernelGGL((k_adj_euler<N, 1>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, gas, out, w.XU, w.Xg);
+    hipLaunchKernelGGL((k_adj_euler<N, 1>), dim3(K), dim3(NT), 0, s, U, m, FU, lam, ub, cub, gas, out, XU, w.Xg, pd);
   else if (riemann == 3)
-    hipLaunchKernelGGL((k_adj_euler<N, 3>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, gas, out, w.XU, w.Xg);
+    hipLaunchKernelGGL((k_adj_euler<N, 3>), dim3(K), dim3(NT), 0, s, U, m, FU, lam, ub, cub, gas, out, XU, w.Xg, pd);
   else
-    hipLaunchKernelGGL((k_adj_euler<N, 33>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, gas, out, w.XU,
-                       w.Xg);
-  hipLaunchKernelGGL((k_adj_visc<N>), dim3(K), dim3(NT), 0, s, U, m, w.FUa, lam, ub, cub, w.Xg, gas, out, w.Xq);
-  if (!stage)
-    hipLaunchKernelGGL((k_adj_gather<N>), dim3(K), dim3(NT), 0, s, U, m, w.XU, w.Xq, gas, out);
-  else
-    hipLaunchKernelGGL((k_adj_stage_update<N>), dim3(K), dim3(NT), 0, s, U, m, w.XU, w.Xq, gas, (const double*)out,
-                       const_cast<double*>(ub), const_cast<double*>(lam), cub, stage[0]);
+    hipLaunchKernelGGL((k_adj_euler<N, 33>), dim3(K), dim3(NT), 0, s, U, m, FU, lam, ub, cub, gas, out, XU, w.Xg,
+                       pd);
+  hipLaunchKernelGGL((k_adj_visc<N>), dim3(K), dim3(NT), 0, s, U, m, FU, (const double*)lam, (const double*)ub, cub,
+                     (const double*)w.Xg, gas, out, XU);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t adj_gather(const DevMesh& m, const double* XU, double* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_adj_gather<N>), dim3(m.nElems), dim3(D<N>::NT), 0, s, m, XU, out);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t adj_update(const DevMesh& m, const double* XU, const double* out, double* ub, double* rbar, double cub,
+                      double a, hipStream_t s) {
+  hipLaunchKernelGGL((k_adj_stage_update<N>), dim3(m.nElems), dim3(D<N>::NT), 0, s, m, XU, out, ub, rbar, cub, a);
   return hipGetLastError();
 }
 
 }  // namespace
 
-namespace {
-hipError_t vjp_dispatch(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                        const double* lam, const double* ub, double cub, double* out, hipStream_t s,
-                        const double* stage, bool faces_ready) {
-  switch (m.N) {
-    case 2: return vjp_n<2>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
-    case 3: return vjp_n<3>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
-    case 4: return vjp_n<4>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
-    case 5: return vjp_n<5>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
-    case 6: return vjp_n<6>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
-    case 7: return vjp_n<7>(m, w, gas, riemann, U, lam, ub, cub, out, s, stage, faces_ready);
-    default: return hipErrorInvalidValue;
+#define FXG_ADJ_SWITCH(CALL)                                                                     \
+  switch (m.N) {                                                                                 \
+    case 2: return CALL(2);                                                                      \
+    case 3: return CALL(3);                                                                      \
+    case 4: return CALL(4);                                                                      \
+    case 5: return CALL(5);                                                                      \
+    case 6: return CALL(6);                                                                      \
+    case 7: return CALL(7);                                                                      \
+    default: return hipErrorInvalidValue;                                                        \
   }
-}
-}  // namespace
 
 hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
                           const double* lam, double* out, hipStream_t s) {
-  return vjp_dispatch(m, w, gas, riemann, U, lam, nullptr, 0.0, out, s, nullptr, false);
+  hipError_t err = launch_faces_of(m, U, w.FUa, s);
+  if (err != hipSuccess) return err;
+  double* lw = const_cast<double*>(lam);  // read only without a pending update
+#define FXG_PARTS(NN) adj_parts<NN>(m, w, gas, riemann, U, w.FUa, lw, nullptr, 0.0, out, w.XU[0], Pending{}, s)
+  err = [&]() -> hipError_t { FXG_ADJ_SWITCH(FXG_PARTS) }();
+#undef FXG_PARTS
+  if (err != hipSuccess) return err;
+#define FXG_GATHER(NN) adj_gather<NN>(m, w.XU[0], out, s)
+  FXG_ADJ_SWITCH(FXG_GATHER)
+#undef FXG_GATHER
 }
 
 hipError_t launch_adjoint_stage(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                                const double* FU, double* rbar, double* ubar, double cub, double a, double* tmp,
-                                hipStream_t s) {
-  const double stage[1] = {a};
-  if (!FU) return vjp_dispatch(m, w, gas, riemann, U, rbar, ubar, cub, tmp, s, stage, false);
-  AdjointWork wf = w;
-  wf.FUa = const_cast<double*>(FU);  // read only by the adjoint kernels
-  return vjp_dispatch(m, wf, gas, riemann, U, rbar, ubar, cub, tmp, s, stage, true);
+                                const double* FU, double* rbar, double* ubar, double cub, double a, double* out,
+                                AdjointChain& ch, hipStream_t s) {
+  if (!FU) {
+    const hipError_t err = launch_faces_of(m, U, w.FUa, s);
+    if (err != hipSuccess) return err;
+    FU = w.FUa;
+  }
+  const Pending pd{ch.pending ? w.XU[ch.cur] : nullptr, ch.cub, ch.a};
+  const int nxt = ch.pending ? 1 - ch.cur : ch.cur;
+#define FXG_PARTS(NN) adj_parts<NN>(m, w, gas, riemann, U, FU, rbar, ubar, cub, out, w.XU[nxt], pd, s)
+  const hipError_t err = [&]() -> hipError_t { FXG_ADJ_SWITCH(FXG_PARTS) }();
+#undef FXG_PARTS
+  ch.cur = nxt;
+  ch.pending = true;
+  ch.cub = cub;
+  ch.a = a;
+  return err;
 }
+
+hipError_t launch_adjoint_flush(const DevMesh& m, const AdjointWork& w, double* rbar, double* ubar,
+                                const double* out, AdjointChain& ch, hipStream_t s) {
+  if (!ch.pending) return hipSuccess;
+  ch.pending = false;
+#define FXG_UPDATE(NN) adj_update<NN>(m, w.XU[ch.cur], out, ubar, rbar, ch.cub, ch.a, s)
+  FXG_ADJ_SWITCH(FXG_UPDATE)
+#undef FXG_UPDATE
+}
+#undef FXG_ADJ_SWITCH
 
 hipError_t launch_axpy(size_t n, double a, const double* x, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, x, y);

@@ -356,8 +356,8 @@ int ensure_adjoint(fxg_ctx* c) {
     return fail("the discrete adjoint is built for the FP64 Gauss-Lobatto split form (N 2..7)");
   if (c->aw.FUa) return 0;
   const size_t face = (size_t)c->nf * c->S, vol = (size_t)c->np * c->K;
-  if (alloc_owned(c, &c->aw.FUa, 10 * face) || alloc_owned(c, &c->aw.XU, 10 * face) ||
-      alloc_owned(c, &c->aw.Xg, 8 * face) || alloc_owned(c, &c->aw.Xq, 8 * face) ||
+  if (alloc_owned(c, &c->aw.FUa, 10 * face) || alloc_owned(c, &c->aw.XU[0], 10 * face) ||
+      alloc_owned(c, &c->aw.XU[1], 10 * face) || alloc_owned(c, &c->aw.Xg, 8 * face) ||
       alloc_owned(c, &c->adjTmp, 5 * vol))
     return 1;
   return 0;
@@ -676,6 +676,9 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
   // U_s = U_{s-1} + RKb_s dt R_s):  rbar += RKb_s dt ubar;  ubar += (dUt/dU)^T rbar;
   // rbar *= -RKA_s (0 for the first stage)
   HIPCHECK(hipMemsetAsync(rbar, 0, bytes, s));
+  // each reverse stage's register update is applied by the next stage's first kernel (the
+  // pending update only needs the exchange arrays, so it also spans the recomputation of a segment)
+  fxg::AdjointChain chain;
   for (int sg = nck - 1; sg >= 0; --sg) {
     const int s0 = sg * every, s1 = (s0 + every < nSteps) ? s0 + every : nSteps;
     const int nst = (s1 - s0) * nStages;
@@ -698,9 +701,10 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
       const int it = s0 + x / nStages, st = x % nStages;
       HIPCHECK(fxg::launch_adjoint_stage(c->m, c->aw, c->gas, c->p.riemann, seg + (size_t)x * vol5,
                                          segF + (size_t)x * face10, rbar, ubar, RKb[st] * dt[it],
-                                         st == 0 ? 0.0 : -RKA[st - 1], c->adjTmp, s));
+                                         st == 0 ? 0.0 : -RKA[st - 1], c->adjTmp, chain, s));
     }
   }
+  HIPCHECK(fxg::launch_adjoint_flush(c->m, c->aw, rbar, ubar, c->adjTmp, chain, s));
   HIPCHECK(hipStreamSynchronize(s));
   c->faces_of = nullptr;
   return 0;

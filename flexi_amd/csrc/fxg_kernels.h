@@ -112,20 +112,35 @@ hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas
 // Scratch of the discrete adjoint of the Gauss-Lobatto split-form right-hand side
 // (fxg_adjoint.hip); face arrays use the FusedWork::FU slot layout.
 struct AdjointWork {
-  double* FUa;  // face states of the linearisation state, 5 per side slot
-  double* XU;   // Riemann-flux adjoint w.r.t. the neighbour's face state, 5 per side slot
-  double* Xg;   // adjoint of the neighbour's viscous normal face flux, 4 per side slot
-  double* Xq;   // lifting adjoint w.r.t. the neighbour's face values (u,v,w,T), 4 per side slot
+  double* FUa;    // face states of the linearisation state, 5 per side slot
+  double* XU[2];  // adjoint parts of the neighbour's face node (Riemann flux and lifting), 5 per
+                  // side slot; two, so that a stage can read the previous stage's while writing
+  double* Xg;     // adjoint of the neighbour's viscous normal face flux, 4 per side slot
+};
+// A previous reverse LSERK stage whose register update is still to be applied (XU == nullptr: none).
+struct Pending {
+  const double* XU;
+  double cub, a;
+};
+// Reverse LSERK stages launched back to back: the register update of a stage is applied by the
+// first kernel of the next one (or by launch_adjoint_flush).
+struct AdjointChain {
+  int cur = 0;
+  bool pending = false;
+  double cub = 0.0, a = 0.0;
 };
 // out = (dUt/dU)^T lam at U (FLEXI layout, FP64, Gauss-Lobatto split form, N 2..7).
 hipError_t launch_rhs_vjp(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
                           const double* lam, double* out, hipStream_t s);
 // One reverse LSERK stage at the stage state U: lam = rbar + cub*ubar; ubar += (dUt/dU)^T lam;
 // rbar = a*lam. `FU`: the face states of U in the fused path's face layout (nullptr: computed
-// into w.FUa). `tmp` is a state-sized scratch array.
+// into w.FUa). `out` is a state-sized scratch array that must stay untouched until the chain's
+// next stage or flush; ubar and rbar are complete only after launch_adjoint_flush.
 hipError_t launch_adjoint_stage(const DevMesh& m, const AdjointWork& w, const Gas& gas, int riemann, const double* U,
-                                const double* FU, double* rbar, double* ubar, double cub, double a, double* tmp,
-                                hipStream_t s);
+                                const double* FU, double* rbar, double* ubar, double cub, double a, double* out,
+                                AdjointChain& chain, hipStream_t s);
+hipError_t launch_adjoint_flush(const DevMesh& m, const AdjointWork& w, double* rbar, double* ubar,
+                                const double* out, AdjointChain& chain, hipStream_t s);
 // y += a*x over n doubles
 hipError_t launch_axpy(size_t n, double a, const double* x, double* y, hipStream_t s);
 
