@@ -448,7 +448,7 @@ __attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) 
       const int me = d == 0 ? i : (d == 1 ? j : k);
       const int stride = d == 0 ? 1 : (d == 1 ? n : n * n);
       const int base = t - me * stride;
-#pragma unroll 1
+#pragma unroll
       for (int l = 0; l <= N; ++l) {
         if (l == me) continue;
         const int id = base + l * stride;
