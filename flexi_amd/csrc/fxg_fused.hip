@@ -252,6 +252,9 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
   }
   double* sM = sX;
   double* sF = sX + 3 * np;
+  // the register R and the state U of the update are loaded before the last direction (whose
+  // arithmetic then hides their latency; fv of the first two directions no longer holds registers)
+  double rpre[5] = {0, 0, 0, 0, 0}, upre[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     RF m0 = 0, m1 = 0, m2 = 0;
@@ -261,6 +264,13 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
       sM[t] = (double)m0; sM[np + t] = (double)m1; sM[2 * np + t] = (double)m2;
 #pragma unroll
       for (int c = 0; c < 4; ++c) sF[c * np + t] = (double)fv[4 * d + c];
+      if (d == 2 && st.mode != 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) upre[v] = U[5 * gid + v];
+        if (st.mode == 2)
+#pragma unroll
+          for (int v = 0; v < 5; ++v) rpre[v] = st.R[5 * gid + v];
+      }
     }
     __syncthreads();
     if (act) {
@@ -309,9 +319,9 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
     } else {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double r = (st.mode == 1) ? ut[v] : st.R[o + v] * st.minusA + ut[v];
+        const double r = (st.mode == 1) ? ut[v] : rpre[v] * st.minusA + ut[v];
         st.R[o + v] = r;
-        un[v] = U[o + v] + r * st.b_dt;
+        un[v] = upre[v] + r * st.b_dt;
       }
 #pragma unroll
       for (int v = 0; v < 5; ++v) st.U[o + v] = un[v];
