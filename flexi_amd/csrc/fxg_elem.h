@@ -79,8 +79,8 @@ __device__ __forceinline__ size_t slot_side_point(unsigned own) {
 // variables held in LDS (sP: u,v,w,T as SoA rows of np), metric transform, surface corrections
 // 0.5*(U_nb - U_own).(n SurfElem)*L_hat at face nodes with the neighbour values staged per own
 // face point (sNb: u,v,w,T as SoA rows of NF), times sJ (lifting_br1.t90:48-181).
-template <int N, class RF>
-__device__ __forceinline__ void lift_node(int t, const double* sP, const double* sB, const double* sNb,
+template <int N, class RF, class LT>
+__device__ __forceinline__ void lift_node(int t, const LT* sP, const double* sB, const LT* sNb,
                                           const RF* Ja, RF sj, RF* g) {
   constexpr int n = N + 1, np = n * n * n, NF = D<N>::NF;
   const BasisLayout B{n};

@@ -38,8 +38,8 @@ __global__ __launch_bounds__(D<N>::NT) void k_visc_faces(const double* __restric
                                                        RF* __restrict__ FG, Gas gas) {
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
   __shared__ double sRaw[5 * np];  // the element's U as stored, loaded with contiguous lanes
-  __shared__ double sP[4 * np];    // u, v, w, T
-  __shared__ double sNb[4 * NF];   // neighbour u, v, w, T per own face point
+  __shared__ RF sP[4 * np];        // u, v, w, T
+  __shared__ RF sNb[4 * NF];       // neighbour u, v, w, T per own face point
   __shared__ unsigned sIdx[NF];
   __shared__ double sB[3 * n * n + 4 * n];
   const int e = blockIdx.x, t = threadIdx.x;
@@ -102,19 +102,23 @@ struct LdsB {
 
 // The Riemann solver is a template parameter: each instance then carries only its own solver's
 // code, which keeps the kernel within 128 VGPRs (two workgroups per CU at N=7) without spills.
+// In FP32-bulk the LDS arrays are FP32 (38 KB at N=7) and the kernel is held to 64 VGPRs, so
+// four workgroups share a CU (six waves per SIMD below N=7, where 8 would spill).
 template <int N, class RF, int SOLVER>
-__global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
+__global__ __launch_bounds__(D<N>::NT)
+__attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
                                                      double* __restrict__ FUnext, const RF* __restrict__ FG,
                                                      Gas gas, double* Ut, RKStage st, RF* __restrict__ Gout) {
   constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
-  // sA rows: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T, 7 E
-  __shared__ double sA[8 * np];
-  __shared__ double sX[LdsB<N>::X];
+  // LDS holds the working precision RF: the values are rounded to RF where they are stored, as
+  // they were rounded where they are read. sA rows: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T, 7 E
+  __shared__ RF sA[8 * np];
+  __shared__ __align__(16) RF sX[LdsB<N>::X];
   __shared__ unsigned sIdx[NF];
   __shared__ double sB[3 * n * n + 4 * n];
   // neighbour rows in sX: 0 u, 1 v, 2 w, 3 T, 4 rho, 5 p, 6 E; surface rows follow
-  double* sNb = sX;
-  double* sSurf = sX + 7 * NF;
+  RF* sNb = sX;
+  RF* sSurf = sX + 7 * NF;
   const BasisLayout B{n};
   const GasT<RF> gt(gas);
   const int e = blockIdx.x, t = threadIdx.x;
@@ -250,8 +254,8 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
     rho = (RF)sA[t]; pp = (RF)sA[np + t]; HH = (RF)sA[2 * np + t];
     uu = (RF)sA[3 * np + t]; vv = (RF)sA[4 * np + t]; ww = (RF)sA[5 * np + t];
   }
-  double* sM = sX;
-  double* sF = sX + 3 * np;
+  RF* sM = sX;
+  RF* sF = sX + 3 * np;
   // the register R and the state U of the update are loaded before the last direction (whose
   // arithmetic then hides their latency; fv of the first two directions no longer holds registers)
   double rpre[5] = {0, 0, 0, 0, 0}, upre[5] = {0, 0, 0, 0, 0};
@@ -261,9 +265,9 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
     if (act) {
       const double* Jg = (d == 0 ? m.Mf : (d == 1 ? m.Mg : m.Mh)) + 3 * gid;
       m0 = (RF)Jg[0]; m1 = (RF)Jg[1]; m2 = (RF)Jg[2];
-      sM[t] = (double)m0; sM[np + t] = (double)m1; sM[2 * np + t] = (double)m2;
+      sM[t] = m0; sM[np + t] = m1; sM[2 * np + t] = m2;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sF[c * np + t] = (double)fv[4 * d + c];
+      for (int c = 0; c < 4; ++c) sF[c * np + t] = fv[4 * d + c];
       if (d == 2 && st.mode != 0) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) upre[v] = U[5 * gid + v];
@@ -343,7 +347,8 @@ __global__ __launch_bounds__(D<N>::NT, sizeof(RF) == 4 ? 4 : 1) void k_dg_fused(
                      m.MetricsVisc + 3 * gid, lam, isnan_rho);
     double mx[6];
     int nanr = 0;
-    block_lambda_max<NT>(lam, isnan_rho, sX, reinterpret_cast<int*>(sX + 6 * (NT / 64)), t, mx, nanr);
+    double* sRd = reinterpret_cast<double*>(sX);
+    block_lambda_max<NT>(lam, isnan_rho, sRd, reinterpret_cast<int*>(sRd + 6 * (NT / 64)), t, mx, nanr);
     if (t == 0) {
       st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
       st.lamSum[2 * e + 1] = mx[3] + mx[4] + mx[5];
