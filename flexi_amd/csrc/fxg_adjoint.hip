@@ -493,16 +493,19 @@ __attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) 
 // XU: the Riemann part of the neighbour's face adjoint (k_adj_euler); the lifting part w.r.t. the
 // neighbour's (u,v,w,T), taken to its conservative variables with its face state, is added here.
 template <int N>
-__global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict__ U, DevMesh m,
+__global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_visc(const double* __restrict__ U, DevMesh m,
                                                      const double* __restrict__ FUa, const double* __restrict__ lam,
                                                      const double* __restrict__ ub, double cub,
                                                      const double* __restrict__ Xg, Gas gas, double* __restrict__ out,
                                                      double* __restrict__ XU) {
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
-  __shared__ double sP[4 * np];    // u, v, w, T
+  // sW: u, v, w, T (rows 0-3) and -sJ*lam of momentum and energy (rows 4-7) until the node
+  // values are formed, then the lifting adjoint per reference derivative (rows 4*d + var)
+  __shared__ double sW[12 * np];
+  double* sP = sW;
+  double* sMu = sW + 4 * np;
+  double* sK = sW;
   __shared__ double sNb[5 * NF];   // neighbour u, v, w, T, rho per own face point
-  __shared__ double sMu[4 * np];   // -sJ*lam, momentum and energy
-  __shared__ double sK[12 * np];   // lifting adjoint per reference derivative: rows 4*d + var
   __shared__ unsigned sIdx[NF];
   __shared__ double sB[3 * n * n + 4 * n];
   const BasisLayout B{n};
@@ -530,6 +533,7 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict_
   const int sides = act ? node_sides<N>(i, j, k) : 0;
   double Ja[9], sj = 0;
   double lq[4] = {0, 0, 0, 0};  // adjoint of u, v, w, T at this node
+  double nu[12];
   if (act) {
     load_metrics<double>(m, gid, Ja, sj);
     double g[12];
@@ -592,10 +596,13 @@ __global__ __launch_bounds__(D<N>::NT) void k_adj_visc(const double* __restrict_
     lg[8] = mu0 * ltxz; lg[2] = mu0 * ltxz;                              // du/dz, dw/dx
     lg[9] = mu0 * ltyz; lg[6] = mu0 * ltyz;                              // dv/dz, dw/dy
     lg[3] = -gas.lambda * lf[0][3]; lg[7] = -gas.lambda * lf[1][3]; lg[11] = -gas.lambda * lf[2][3];
-    // lifting adjoint: nu = sJ*lg; reference-derivative adjoint kappa^d = Ja^d . nu
-    double nu[12];
+    // lifting adjoint: nu = sJ*lg
 #pragma unroll
     for (int c = 0; c < 12; ++c) nu[c] = sj * lg[c];
+  }
+  __syncthreads();  // sP and sMu are no longer read: sK takes their place
+  if (act) {
+    // reference-derivative adjoint kappa^d = Ja^d . nu
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
