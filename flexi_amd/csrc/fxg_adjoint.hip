@@ -65,7 +65,8 @@ __device__ __forceinline__ RotState<Dual> rot_state(const Dual* c, double kappaM
 // Reverse of riemann_rot (split form) at one side point: lF is the adjoint of F in the side
 // frame; the adjoints of the two rotated states' (rho, E, p, vn, vt1, vt2) are returned in aL, aR.
 // Line by line the transpose of the forward arithmetic in fxg_physics.h (riemann.f90:558-855,
-// splitflux.f90:672-700); srho = 1/rho is folded into rho.
+// splitflux.f90:672-700); srho = 1/rho is folded into rho. Divisions are taken as products with
+// reciprocals formed once (1/rho, 1/c, 1/sqrt(rho)): the adjoint's rounding needs no match.
 struct RotAdj {
   double rho, E, p, vn, vt1, vt2;
 };
@@ -77,7 +78,7 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
   aR = RotAdj{0, 0, 0, 0, 0, 0};
   // central part, Pirozzoli surface flux: F += Fc
   {
-    const double HL = (L.E + L.p) / L.rho, HR = (R.E + R.p) / R.rho;
+    const double HL = (L.E + L.p) * L.srho, HR = (R.E + R.p) * R.srho;
     const double rs = L.rho + R.rho, us = L.vn + R.vn, sv1 = L.vt1 + R.vt1, sv2 = L.vt2 + R.vt2, Hs = HL + HR;
     const double lrs = 0.25 * us * lF[0] + 0.125 * us * us * lF[1] + 0.125 * us * sv1 * lF[2] +
                        0.125 * us * sv2 * lF[3] + 0.125 * Hs * us * lF[4];
@@ -85,12 +86,12 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
                        0.125 * rs * sv2 * lF[3] + 0.125 * rs * Hs * lF[4];
     const double lsv1 = 0.125 * rs * us * lF[2], lsv2 = 0.125 * rs * us * lF[3], lHs = 0.125 * rs * us * lF[4];
     aL.p += 0.5 * lF[1]; aR.p += 0.5 * lF[1];
-    aL.rho += lrs - HL * lHs / L.rho; aR.rho += lrs - HR * lHs / R.rho;
+    aL.rho += lrs - HL * lHs * L.srho; aR.rho += lrs - HR * lHs * R.srho;
     aL.vn += lus; aR.vn += lus;
     aL.vt1 += lsv1; aR.vt1 += lsv1;
     aL.vt2 += lsv2; aR.vt2 += lsv2;
-    aL.E += lHs / L.rho; aR.E += lHs / R.rho;
-    aL.p += lHs / L.rho; aR.p += lHs / R.rho;
+    aL.E += lHs * L.srho; aR.E += lHs * R.srho;
+    aL.p += lHs * L.srho; aR.p += lHs * R.srho;
   }
   const double UL[5] = {L.rho, L.rho * L.vn, L.rho * L.vt1, L.rho * L.vt2, L.E};
   const double UR[5] = {R.rho, R.rho * R.vn, R.rho * R.vt1, R.rho * R.vt2, R.E};
@@ -118,13 +119,14 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
     // Roe averages
     const double HLr = (L.E + L.p) * L.srho, HRr = (R.E + R.p) * R.srho;
     const double sqL = sqrt(L.rho), sqR = sqrt(R.rho);
-    const double ss = 1.0 / (sqL + sqR);
+    const double ss = 1.0 / (sqL + sqR), isqL = 1.0 / sqL, isqR = 1.0 / sqR;
     const double ru = (sqR * R.vn + sqL * L.vn) * ss;
     const double rv = (sqR * R.vt1 + sqL * L.vt1) * ss;
     const double rw = (sqR * R.vt2 + sqL * L.vt2) * ss;
     const double absVel = ru * ru + rv * rv + rw * rw;
     const double H = (sqR * HRr + sqL * HLr) * ss;
     const double c = sqrt(g.kappaM1 * (H - 0.5 * absVel));
+    const double ic = 1.0 / c;
     double al[5], a[5];
     double d[5];
     for (int k = 0; k < 5; ++k) d[k] = UR[k] - UL[k];
@@ -134,8 +136,8 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
       const double d5 = d[4] - (d[2] - rv * d[0]) * rv - (d[3] - rw * d[0]) * rw;
       al[2] = d[2] - rv * d[0];
       al[3] = d[3] - rw * d[0];
-      al[1] = g.kappaM1 / (c * c) * (d[0] * (H - ru * ru) - d5 + ru * d[1]);
-      al[0] = 0.5 / c * (d[0] * (ru + c) - d[1] - c * al[1]);
+      al[1] = g.kappaM1 * ic * ic * (d[0] * (H - ru * ru) - d5 + ru * d[1]);
+      al[0] = 0.5 * ic * (d[0] * (ru + c) - d[1] - c * al[1]);
       al[4] = d[0] - al[0] - al[1];
       a[0] = fabs(ru - c); a[1] = a[2] = a[3] = fabs(ru); a[4] = fabs(ru + c);
     } else {
@@ -143,7 +145,7 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
       rd = sqrt(L.rho * R.rho);
       const double d1 = R.rho - L.rho, dv = R.vt1 - L.vt1, dw = R.vt2 - L.vt2;
       du = R.vn - L.vn; dp = R.p - L.p;
-      tmp = 0.5 / (c * c);
+      tmp = 0.5 * ic * ic;
       al[0] = tmp * (dp - rd * c * du);
       al[1] = d1 - dp * 2.0 * tmp;
       al[2] = rd * dv;
@@ -183,14 +185,12 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
       // al4 = d0 - al0 - al1
       ld[0] += lal[4]; lal[0] -= lal[4]; lal[1] -= lal[4];
       // al0 = 0.5/c*Y, Y = d0(ru+c) - d1 - c al1
-      const double Y = d[0] * (ru + c) - d[1] - c * al[1];
-      const double lY = 0.5 * lal[0] / c;
-      lc -= al[0] / c * lal[0];
+      const double lY = 0.5 * lal[0] * ic;
+      lc -= al[0] * ic * lal[0];
       ld[0] += (ru + c) * lY; lru += d[0] * lY; lc += (d[0] - al[1]) * lY; ld[1] -= lY; lal[1] -= c * lY;
-      (void)Y;
       // al1 = kappaM1/c^2*X, X = d0(H - ru^2) - d5 + ru d1
-      const double lX = g.kappaM1 * lal[1] / (c * c);
-      lc -= 2.0 * al[1] / c * lal[1];
+      const double lX = g.kappaM1 * lal[1] * ic * ic;
+      lc -= 2.0 * al[1] * ic * lal[1];
       double ld5 = -lX;
       ld[0] += (H - ru * ru) * lX; lH += d[0] * lX; lru += (d[1] - 2.0 * ru * d[0]) * lX; ld[1] += ru * lX;
       // al2 = d2 - rv d0, al3 = d3 - rw d0
@@ -244,8 +244,9 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
       const double ldu = -tmp * rd * c * lal[0] + tmp * rd * c * lal[4];
       const double ld1 = lal[1];
       const double ldv = rd * lal[2], ldw = rd * lal[3];
-      lc -= ltmp / (c * c * c);  // tmp = 0.5/c^2
-      aL.rho += R.rho / (2.0 * rd) * lrd; aR.rho += L.rho / (2.0 * rd) * lrd;
+      lc -= ltmp * ic * ic * ic;  // tmp = 0.5/c^2
+      const double hrd = 0.5 / rd;
+      aL.rho += R.rho * hrd * lrd; aR.rho += L.rho * hrd * lrd;
       aR.rho += ld1; aL.rho -= ld1;
       aR.vn += ldu; aL.vn -= ldu;
       aR.vt1 += ldv; aL.vt1 -= ldv;
@@ -255,8 +256,8 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
       sound_adj(R, cR, lcR, aR);
     }
     // c = sqrt(kappaM1 (H - absVel/2))
-    lH += 0.5 * g.kappaM1 / c * lc;
-    labs -= 0.25 * g.kappaM1 / c * lc;
+    lH += 0.5 * g.kappaM1 * ic * lc;
+    labs -= 0.25 * g.kappaM1 * ic * lc;
     lru += 2.0 * ru * labs; lrv += 2.0 * rv * labs; lrw += 2.0 * rw * labs;
     // H, ru, rv, rw: Roe averages with weights sqL*ss, sqR*ss
     double lsqL = HLr * ss * lH + L.vn * ss * lru + L.vt1 * ss * lrv + L.vt2 * ss * lrw;
@@ -270,7 +271,7 @@ __device__ __forceinline__ void riemann_rot_vjp(const Gas& g, const RotState<dou
     const double lHL = sqL * ss * lH, lHR = sqR * ss * lH;
     aL.E += L.srho * lHL; aL.p += L.srho * lHL; aL.rho -= (L.E + L.p) * L.srho * L.srho * lHL;
     aR.E += R.srho * lHR; aR.p += R.srho * lHR; aR.rho -= (R.E + R.p) * R.srho * R.srho * lHR;
-    aL.rho += lsqL / (2.0 * sqL); aR.rho += lsqR / (2.0 * sqR);
+    aL.rho += 0.5 * lsqL * isqL; aR.rho += 0.5 * lsqR * isqR;
   }
   // U = (rho, rho vn, rho vt1, rho vt2, E) of both sides through d = UR - UL
   aR.rho += ld[0] + R.vn * ld[1] + R.vt1 * ld[2] + R.vt2 * ld[3];
