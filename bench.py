@@ -35,17 +35,18 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 #   visc_faces : U 512*40 + boundary metrics 296*80 + neighbour face states 384*40
 #                + own viscous fluxes written 384*32                              -> 140.25 B/DOF
 #   dg_fused   : U 40 + metrics 80 + neighbour states 30 + neighbour fluxes 24 + U written 40
-#                + next face states 30 + register R read 40 (not in stage 1) + written 40,
-#                averaged over the 5 stages                                       -> 316.0 B/DOF
+#                + next face states 30 + register R read 40 (not in stage 1) + written 40
+#                + MetricsAdv/MetricsVisc 48 (stage 5: CALCTIMESTEP's spectral radii),
+#                averaged over the 5 stages: (284 + 3*324 + 372)/5               -> 325.6 B/DOF
 # The four-kernel path's figures follow the same counting.
 ALGO_BYTES_PER_DOF = {"prolong_u": 60.0, "lift": 372.0, "flux": 147.0, "volint": 366.0,
-                      "faces": 53.125, "visc_faces": 140.25, "dg_fused": 316.0}
+                      "faces": 53.125, "visc_faces": 140.25, "dg_fused": 325.6}
 # Per DOF-stage byte model of the stage (U, Ut/R, metrics and gradient traffic of one stage)
 B_STAGE = 432.0
-# HBM bytes per DOF of one launch of the fused kernels at this workload (FP64), from rocprofv3
-# FETCH_SIZE (x2.0, plain-copy calibration) + WRITE_SIZE runs of this benchmark; the summary and
-# the commands are in profiles/fused_stage_tgv32_n7_fp64.md
-MEASURED_BYTES_PER_DOF = {"dg_fused": 380.8, "visc_faces": 174.3, "faces": 61.3}
+# HBM bytes per DOF of one launch of the fused kernels at this workload (FP64, mean over the 5
+# stages of a step), from rocprofv3 FETCH_SIZE (x2.0, plain-copy calibration) + WRITE_SIZE runs of
+# this benchmark; the summary and the commands are in profiles/fused_stage_tgv32_n7_fp64.md
+MEASURED_BYTES_PER_DOF = {"dg_fused": 431.1, "visc_faces": 140.6, "faces": 61.3}
 
 
 def make_solver(nel, precision):
@@ -237,7 +238,12 @@ def main():
     ap.add_argument("--no-curvilinear", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (calibrates HBM counters)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the adjoint/forward measurement and print its JSON")
     args = ap.parse_args()
+    if args.adjoint_only:
+        print(json.dumps(adjoint_ratio()), flush=True)
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe // 8
