@@ -37,7 +37,32 @@ def _compile(src: str) -> str:
     return obj
 
 
+# Mesh and state files (include/fxg_io.h): host code over the HDF5 C library, a separate library
+# so that libfxg.so does not depend on HDF5.
+HDF5_ROOT = os.environ.get("FXG_HDF5_ROOT", "/opt/conda")
+IO_LIB = os.path.join(HERE, "libfxg_io.so")
+
+
+def build_io(verbose: bool = False) -> str | None:
+    """libfxg_io.so with g++ against HDF5_ROOT; None (and a note) when HDF5 is not installed."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"fxg_io not built: no hdf5.h under {inc} (set FXG_HDF5_ROOT)", file=sys.stderr)
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc, os.path.join(CSRC, "fxg_io.cpp"),
+           "-o", tmp, "-L", lib, "-lhdf5", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for fxg_io.cpp:\n{r.stderr[-8000:]}")
+    os.replace(tmp, IO_LIB)
+    if verbose:
+        print("built", IO_LIB)
+    return IO_LIB
+
+
 def build(verbose: bool = False) -> str:
+    build_io(verbose)
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))

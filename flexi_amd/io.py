@@ -1,0 +1,97 @@
+"""FLEXI's mesh (HOPR) and State files, through libfxg_io.so (include/fxg_io.h).
+
+``read_mesh(path)`` gives the same ``Mesh`` as ``flexi_amd.mesh.periodic_box`` (FLEXI's
+SideToElem / ElemToSide / NodeCoords after ReadMesh, setLocalSideIDs and fillMeshInfo,
+src/mesh/mesh_readin.f90:197-560, src/mesh/prepare_mesh.f90:50-737) for conforming hexahedral
+meshes whose sides are all inner or periodic. ``write_state`` / ``read_state`` write and read
+FLEXI State files (DG_Solution(5,0:N,0:N,0:N,nElems) and the header attributes of
+src/io_hdf5/hdf5_output.f90:741-960), so a FLEXI run can restart from a state written here and
+the other way round.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .mesh import Mesh
+
+IO_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfxg_io.so")
+_lib = None
+
+# FLEXI's NodeType strings (src/interpolation/interpolation_vars.f90) for the solver's names
+NODE_TYPE_NAMES = {"gauss": "GAUSS", "gauss-lobatto": "GAUSS-LOBATTO"}
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(IO_LIB_PATH):
+            raise RuntimeError(f"{IO_LIB_PATH} is not built: run python -m flexi_amd.build "
+                               f"(needs the HDF5 C library, FXG_HDF5_ROOT)")
+        L = C.CDLL(IO_LIB_PATH)
+        ip, dp, cp = C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_char_p
+        L.fxg_io_last_error.restype = cp
+        L.fxg_io_mesh_size.argtypes = [cp, ip, ip, ip]
+        L.fxg_io_read_mesh.argtypes = [cp, C.c_int, C.c_int, C.c_int, ip, ip, dp]
+        L.fxg_io_state_size.argtypes = [cp, ip, ip, ip]
+        L.fxg_io_read_state.argtypes = [cp, C.c_int, C.c_int, C.c_int, dp, dp, cp, C.c_int, cp, C.c_int]
+        L.fxg_io_write_state.argtypes = [cp, cp, cp, cp, C.c_int, C.c_int, C.c_double, dp]
+        for f in ("fxg_io_mesh_size", "fxg_io_read_mesh", "fxg_io_state_size", "fxg_io_read_state",
+                  "fxg_io_write_state"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(lib().fxg_io_last_error().decode())
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def read_mesh(path: str) -> Mesh:
+    """A HOPR mesh file as FLEXI's connectivity and geometry arrays (Fortran order, 1-based)."""
+    L = lib()
+    K, S, NGeo = C.c_int(), C.c_int(), C.c_int()
+    _check(L.fxg_io_mesh_size(path.encode(), C.byref(K), C.byref(S), C.byref(NGeo)))
+    K, S, NGeo = K.value, S.value, NGeo.value
+    s2e = np.zeros((5, S), dtype=np.int32, order="F")
+    e2s = np.zeros((2, 6, K), dtype=np.int32, order="F")
+    nodes = np.zeros((3, NGeo + 1, NGeo + 1, NGeo + 1, K), dtype=np.float64, order="F")
+    _check(L.fxg_io_read_mesh(path.encode(), K, S, NGeo, _ptr(s2e, C.c_int), _ptr(e2s, C.c_int),
+                              _ptr(nodes, C.c_double)))
+    lo = nodes.reshape(3, -1).min(axis=1)
+    hi = nodes.reshape(3, -1).max(axis=1)
+    return Mesh(nElems=K, nSides=S, NGeo=NGeo, NodeCoords=nodes, ElemToSide=e2s, SideToElem=s2e,
+                shape=None, box_lo=tuple(float(x) for x in lo), box_len=tuple(float(x) for x in hi - lo))
+
+
+def write_state(path: str, U: np.ndarray, N: int, node_type: str, time: float, mesh_file: str,
+                project: str = "fxg") -> None:
+    """U(5,0:N,0:N,0:N,nElems) (Fortran order) as a FLEXI State file."""
+    U = np.asfortranarray(U, dtype=np.float64)
+    if U.ndim != 5 or U.shape[0] != 5 or U.shape[1:4] != (N + 1,) * 3:
+        raise ValueError(f"U must be (5,{N + 1},{N + 1},{N + 1},nElems), got {U.shape}")
+    nt = NODE_TYPE_NAMES.get(node_type.lower(), node_type)
+    _check(lib().fxg_io_write_state(path.encode(), project.encode(), mesh_file.encode(), nt.encode(), N,
+                                    U.shape[4], float(time), _ptr(U, C.c_double)))
+
+
+def read_state(path: str) -> dict:
+    """A FLEXI State file: {'U': (nVar,N+1,N+1,N+1,nElems) Fortran order, 'N', 'time',
+    'node_type', 'mesh_file'}."""
+    L = lib()
+    nv, N, K = C.c_int(), C.c_int(), C.c_int()
+    _check(L.fxg_io_state_size(path.encode(), C.byref(nv), C.byref(N), C.byref(K)))
+    nv, N, K = nv.value, N.value, K.value
+    U = np.zeros((nv, N + 1, N + 1, N + 1, K), dtype=np.float64, order="F")
+    t = C.c_double()
+    nt = C.create_string_buffer(256)
+    mf = C.create_string_buffer(256)
+    _check(L.fxg_io_read_state(path.encode(), nv, N, K, _ptr(U, C.c_double), C.byref(t), nt, 256, mf, 256))
+    return {"U": U, "N": N, "time": t.value, "node_type": nt.value.decode(), "mesh_file": mf.value.decode()}

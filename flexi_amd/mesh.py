@@ -91,9 +91,9 @@ class Mesh:
     NodeCoords: np.ndarray   # (3,NGeo+1,NGeo+1,NGeo+1,nElems) float64
     ElemToSide: np.ndarray   # (2,6,nElems) int32, 1-based SideID
     SideToElem: np.ndarray   # (5,nSides) int32, 1-based ElemIDs
-    shape: tuple[int, int, int]
-    box_lo: tuple[float, float, float]
-    box_len: tuple[float, float, float]
+    shape: Optional[tuple[int, int, int]] = None       # elements per direction of a generated box
+    box_lo: Optional[tuple[float, float, float]] = None  # bounding box of the nodes
+    box_len: Optional[tuple[float, float, float]] = None
 
     @property
     def firstInnerSide(self) -> int:

@@ -1,6 +1,8 @@
 """GPU checks of whole runs: FP32-bulk accuracy against the FP64 run (Taylor-Green vortex, 8^3,
 N=3 Gauss, weak form, LF, 1000 steps) and freestream preservation on the curvilinear 24^3 N=5
 mesh with metrics computed on the GPU."""
+import os
+
 import numpy as np
 import pytest
 
@@ -80,18 +82,24 @@ def test_curvilinear_freestream_24cubed_n5_gpu_metrics(pipeline, monkeypatch):
     assert err <= (1e-13 if pipeline == "fused" else 1e-11)
 
 
-def test_gpu_reproduces_flexi_tgv_split_regression_trajectory():
+@pytest.mark.parametrize("mesh_source", ["box", "hopr_file"])
+def test_gpu_reproduces_flexi_tgv_split_regression_trajectory(mesh_source):
     """regressioncheck/checks/tgv/split of the reference: N=7 GL, split PI, RoeEntropyFix, BR1,
     8^3 on [0,2pi]^3, CFL=DFL=0.8, TGV analysis every 10 steps. The GPU state and lifted
     gradients are analysed with the oracle's AnalyzeTestcase restatement and compared with the
     reference CSV over the first 50 steps (the reference's own tolerance is 1e-4 relative). The
     gradients are those of the last stage's right-hand side, which the library recomputes from
-    U - RKb_5*dt*R for the fused path."""
+    U - RKb_5*dt*R for the fused path. The mesh is the generated box or the case's own HOPR file
+    (CART_HEX_PERIODIC_008_mesh.h5, read by flexi_amd.io with FLEXI's side numbering)."""
     import flexi_fixtures as fx
     import oracle as orc
     from flexi_amd.solver import DGSolver
     header, data = fx.tgv_split_csv()
-    mesh = periodic_box(8, 8, 8, lo=(0, 0, 0))
+    if mesh_source == "box":
+        mesh = periodic_box(8, 8, 8, lo=(0, 0, 0))
+    else:
+        from flexi_amd import io as fio
+        mesh = fio.read_mesh(os.path.join(os.path.dirname(__file__), "golden", "hdf5", "CART_HEX_PERIODIC_008_mesh.h5"))
     m = orc.calc_metrics(7, orc.NODE_GL, mesh)
     S = orc.Solver(orc.params(7, orc.NODE_GL, 1, orc.RIEMANN_ROEENTROPYFIX, mu0=6.25e-4, R=71.42857), mesh, m)
     g = DGSolver(mesh, 7, node_type="gauss-lobatto", split_dg="pi", riemann="roeentropyfix", metrics=m,
