@@ -58,6 +58,33 @@ def test_fp32_bulk_tgv_1000_steps_against_fp64():
     assert err_eps <= 1e-3
 
 
+@pytest.mark.parametrize("N", [5, 7])
+def test_fp32_bulk_fused_split_against_fp64(N):
+    """The fused Gauss-Lobatto split-form stage in FP32-bulk (FP32 LDS, FP32 Riemann/split/
+    lifting arithmetic; FP64 state, register, metrics) against FP64: TGV 8^3, split PI, Roe,
+    density perturbed by 1e-3, 300 steps of the FP64 run's first dt."""
+    from flexi_amd.solver import DGSolver
+    mesh = periodic_box(8, 8, 8)
+    runs, dt = {}, None
+    for prec in ("fp64", "fp32-bulk"):
+        s = DGSolver(mesh, N, node_type="gauss-lobatto", split_dg="pi", riemann="roe", precision=prec,
+                     CFLScale=0.8, DFLScale=0.8)
+        assert s.kernel_pipeline() == "fused"
+        s.set_state(testcase.tgv_density_perturbed(s.metrics()["Elem_xGP"], amplitude=1e-3, seed=1))
+        if dt is None:
+            dt, err = s.CalcTimeStep()
+            assert err == 0
+        s.run(300, fixed_dt=dt)
+        runs[prec] = s.get_state()
+        s.close()
+        assert np.all(np.isfinite(runs[prec])), prec
+    U64, U32 = runs["fp64"], runs["fp32-bulk"]
+    err_U = np.linalg.norm((U32 - U64).ravel()) / np.linalg.norm(U64.ravel())
+    print(f"fused FP32-bulk N={N} after 300 steps: rel L2 U {err_U:.3e}")
+    assert np.all(np.isfinite(U32))
+    assert err_U <= 1e-6
+
+
 @pytest.mark.parametrize("pipeline", ["fused", "classic"])
 def test_curvilinear_freestream_24cubed_n5_gpu_metrics(pipeline, monkeypatch):
     from flexi_amd.solver import DGSolver
