@@ -43,10 +43,10 @@ def setup(N, node, split, riemann, shape=(3, 3, 3), deform=None, NGeo=1, lo=(-np
 
 CASES = [
     # (N, nodeType, split, riemann)
-    (3, orc.NODE_G, False, orc.RIEMANN_LF),        # configuration 1
+    (3, orc.NODE_G, False, orc.RIEMANN_LF),        # the plain method (Gauss, weak form, LF)
     (3, orc.NODE_G, False, orc.RIEMANN_ROE),
     (5, orc.NODE_GL, True, orc.RIEMANN_ROE),
-    (7, orc.NODE_GL, True, orc.RIEMANN_ROE),       # configuration 2
+    (7, orc.NODE_GL, True, orc.RIEMANN_ROE),       # the 32^3 throughput scheme (GL, split PI, Roe)
     (7, orc.NODE_GL, True, orc.RIEMANN_ROEENTROPYFIX),
     (4, orc.NODE_GL, True, orc.RIEMANN_LF),
     (5, orc.NODE_GL, False, orc.RIEMANN_ROEENTROPYFIX),
