@@ -199,5 +199,27 @@ __device__ __forceinline__ int bnd_rank(int i, int j, int k) {
   const int ring = (j == 0) ? i : (j == N ? n + 2 * (N - 1) + i : n + 2 * (j - 1) + (i == 0 ? 0 : 1));
   return nf + (k - 1) * 4 * N + ring;
 }
+// Inverse of bnd_rank: the node (i,j,k) of boundary rank b, 0 <= b < Bnd<N>::NB.
+template <int N>
+__device__ __forceinline__ void bnd_node(int b, int& i, int& j, int& k) {
+  constexpr int n = N + 1, nf = n * n, NR = (N - 1) * 4 * N;
+  if (b < nf) {
+    k = 0; i = b % n; j = b / n;
+  } else if (b >= nf + NR) {
+    const int r = b - nf - NR;
+    k = N; i = r % n; j = r / n;
+  } else {
+    const int r = b - nf, ring = r % (4 * N);
+    k = 1 + r / (4 * N);
+    if (ring < n) {
+      j = 0; i = ring;
+    } else if (ring >= n + 2 * (N - 1)) {
+      j = N; i = ring - n - 2 * (N - 1);
+    } else {
+      const int q = ring - n;
+      j = 1 + q / 2; i = (q % 2) ? N : 0;
+    }
+  }
+}
 
 }  // namespace fxg

@@ -1,7 +1,7 @@
 // Fused two-kernel stage for Gauss-Lobatto nodes and the split form (the benchmark path).
 //
 // Per stage, instead of prolongation / lifting / side fluxes / volume integral:
-//   A  k_visc_faces: per element, the BR1 gradients at its boundary nodes and, for each of its
+//   A  k_visc_faces (fxg_visc.hip): per element, the BR1 gradients at its boundary nodes and, for each of its
 //      six faces, the viscous normal flux f_v . (n SurfElem) seen from that element (4 values per
 //      face point, momentum and energy) into the side array FG.
 //   B  k_dg_fused:   per element, the face fluxes of its six sides (Riemann solver with the
@@ -29,77 +29,6 @@
 
 namespace fxg {
 namespace {
-
-// ---------------------------------------------------------------------------------------------
-// Kernel A: viscous normal fluxes of the element's faces.
-// FP64 stages the element's U in LDS with contiguous lanes (77 VGPRs, 52 KB: three workgroups
-// per CU at N=7); FP32-bulk converts U from global memory directly and is held to 64 VGPRs
-// (18 KB: four workgroups per CU, 0.44 ms instead of 0.53 ms at 32^3, N=7).
-template <int N, class RF>
-__global__ __launch_bounds__(D<N>::NT)
-__attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) void k_visc_faces(
-    const double* __restrict__ U, DevMesh m, const double* __restrict__ Jsoa, const double* __restrict__ FU,
-    RF* __restrict__ FG, Gas gas) {
-  constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
-  constexpr bool raw = sizeof(RF) == 8;
-  __shared__ double sRaw[raw ? 5 * np : 1];  // the element's U as stored, loaded with contiguous lanes
-  __shared__ RF sP[4 * np];        // u, v, w, T
-  __shared__ RF sNb[4 * NF];       // neighbour u, v, w, T per own face point
-  __shared__ unsigned sIdx[NF];
-  __shared__ double sB[3 * n * n + 4 * n];
-  const int e = blockIdx.x, t = threadIdx.x;
-  const double* Ue = U + (size_t)5 * np * e;
-  if (raw)
-    for (int x = t; x < 5 * np; x += NT) sRaw[x] = Ue[x];
-  else if (t < np) {
-    const Prim q = cons_to_prim(Ue + 5 * t, gas);
-    sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
-  }
-  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
-  for (int x = t; x < NF; x += NT) {
-    const unsigned own = face_slot<N>(m, e, x);
-    sIdx[x] = own;
-    const Prim q = cons_to_prim(FU + 5 * (size_t)slot_nbr<N>(own), gas);
-    sNb[x] = q.u; sNb[NF + x] = q.v; sNb[2 * NF + x] = q.w; sNb[3 * NF + x] = q.T;
-  }
-  __syncthreads();
-  if (raw) {
-    if (t < np) {
-      const Prim q = cons_to_prim(sRaw + 5 * t, gas);
-      sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
-    }
-    __syncthreads();
-  }
-  if (t >= np) return;
-  const int i = t % n, j = (t / n) % n, k = t / (n * n);
-  const int sides = node_sides<N>(i, j, k);
-  if (!sides) return;
-  const GasT<RF> gt(gas);
-  // metric terms of the boundary nodes only, rows of NB per element (Jsoa layout)
-  constexpr int NB = Bnd<N>::NB;
-  const double* Je = Jsoa + (size_t)10 * NB * e;
-  RF Ja[9], sj;
-  {
-    const int b = bnd_rank<N>(i, j, k);
-#pragma unroll
-    for (int r = 0; r < 9; ++r) Ja[r] = (RF)Je[r * NB + b];
-    sj = (RF)Je[9 * NB + b];
-  }
-  RF g[12];
-  lift_node<N, RF>(t, sP, sB, sNb, Ja, sj, g);
-  RF fx[4], fy[4], fz[4];
-  diff_flux<RF>(gt, (RF)sP[t], (RF)sP[np + t], (RF)sP[2 * np + t], g, g + 4, g + 8, fx, fy, fz);
-#pragma unroll 1
-  for (int ls = 1; ls <= 6; ++ls) {
-    if (!(sides & (1 << (ls - 1)))) continue;
-    const int dir = dir_of(ls);
-    const RF s = (RF)normal_sign(ls);
-    const RF n0 = Ja[3 * dir] * s, n1 = Ja[3 * dir + 1] * s, n2 = Ja[3 * dir + 2] * s;
-    RF* dst = FG + 4 * (size_t)sIdx[face_point<N>(ls, i, j, k)];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) dst[c] = fx[c] * n0 + fy[c] * n1 + fz[c] * n2;
-  }
-}
 
 // ---------------------------------------------------------------------------------------------
 // Kernel B: the rest of the stage.
@@ -413,7 +342,8 @@ hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int rie
   }
   {
     Scope sc(tm, K_VISC_FACES, s);
-    hipLaunchKernelGGL((k_visc_faces<N, RF>), dim3(K), dim3(NT), 0, s, U, m, w.Jsoa, w.FU[cur], (RF*)w.FG, gas);
+    const hipError_t err = launch_visc_faces(m, sizeof(RF) == 4, U, w.Jsoa, w.FU[cur], w.FG, gas, s);
+    if (err != hipSuccess) return err;
   }
   {
     Scope sc(tm, K_DG_FUSED, s);

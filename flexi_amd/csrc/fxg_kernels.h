@@ -105,6 +105,9 @@ bool fused_supported(int N, int nodeType, int split, long long nSides);
 // One stage of the fused path. FU[cur] must hold the face states of U unless faces_fresh is
 // false, in which case they are rebuilt first. In RK modes the new state's faces go to FU[1-cur].
 // G (optional, (np,12,nElems) of the gradient type) receives the lifted gradients.
+// Kernel A of the fused stage (fxg_visc.hip): viscous normal face fluxes into FG (float if fp32).
+hipError_t launch_visc_faces(const DevMesh& m, bool fp32, const double* U, const double* Jsoa, const double* FU,
+                             void* FG, const Gas& gas, hipStream_t s);
 hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, int fp32,
                             const double* U, double* Ut, const RKStage& st, bool faces_fresh, int cur,
                             void* G, hipStream_t s, KernelTimer* timer);
