@@ -10,13 +10,17 @@
 // Reverse:
 //   k_adj_euler  split-flux adjoint (each pair's flux depends on sums of the two nodes' primitive
 //                variables, so one pair Jacobian serves both nodes), Riemann-flux adjoint per own
-//                face point (directional derivatives with dual numbers, 10 per point), the
-//                primitive-to-conservative chain; writes out and, per own face point, the parts
-//                that belong to the neighbour: XU (its face state) and Xg (its viscous face flux).
+//                face point (the hand-written reverse of riemann_rot; with -DFXG_ADJ_DUAL=1
+//                directional derivatives in dual numbers instead), the primitive-to-conservative
+//                chain; writes out and, per own face point, the parts that belong to the
+//                neighbour: XU (its face state) and Xg (its viscous face flux). In the reverse
+//                stepper it first applies the previous reverse stage's register update.
 //   k_adj_visc   BR1 gradients recomputed; viscous-flux adjoint (volume, own face half, the
 //                neighbour's half from Xg); lifting adjoint (transposed line derivatives and face
-//                corrections); adds to out and writes Xq (the neighbour's face-state part).
-//   k_adj_gather adds XU and Xq of the neighbours at the face nodes.
+//                corrections); adds to out, and adds the neighbour's lifting part, taken to its
+//                conservative variables, to XU.
+//   k_adj_gather adds the neighbours' XU at the face nodes (k_adj_stage_update: and the register
+//                update of the last reverse stage).
 // References for the forward terms: src/dg/volint.f90:211-371, src/equations/navierstokes/
 // splitflux.f90:603-700, riemann.f90:267-855, fillflux.f90:40-215, surfint.t90:471-616,
 // lifting_br1.t90:48-292, flux.f90:149-241.
