@@ -1,6 +1,7 @@
 // Host side of libfxg: context set-up, the DG basis, device memory and the C ABI of include/fxg.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -170,6 +171,11 @@ struct fxg_ctx {
   // discrete adjoint scratch, allocated on first use
   fxg::AdjointWork aw{};
   double* adjTmp = nullptr;
+  // boundary sides 1..nBC: their conditions must be set (fxg_set_boundaries) before a right-hand side
+  int nBC = 0;
+  bool bcSet = false;
+  int* bcInfo = nullptr;
+  double* refPrim = nullptr;
 };
 
 namespace {
@@ -190,12 +196,15 @@ int check_params(const fxg_params* p) {
   return 0;
 }
 
-// Connectivity must be a closed conforming mesh: every side has one master (flip 0) and one
-// slave, consistent between SideToElem and ElemToSide. Checked on the host so that no kernel
-// can index out of bounds.
-int check_connectivity(int K, int S, const int* S2E, const int* E2S) {
+// Connectivity must be a conforming mesh: every side has one master (flip 0) and one slave,
+// consistent between SideToElem and ElemToSide, except the boundary sides, which have a master
+// only (SideToElem(2,:) = -1) and come first, 1..nBC, as setLocalSideIDs numbers them
+// (prepare_mesh.f90:196-222). Checked on the host so that no kernel can index out of bounds.
+int check_connectivity(int K, int S, const int* S2E, const int* E2S, int* nBCOut = nullptr) {
   if (K <= 0 || S <= 0) return fail("nElems and nSides must be positive");
   if (!S2E || !E2S) return fail("SideToElem/ElemToSide is NULL");
+  int nBC = 0;
+  while (nBC < S && S2E[5 * nBC + 1] == -1) ++nBC;
   std::vector<int> seen(2 * (size_t)S, 0);
   for (int e = 0; e < K; ++e)
     for (int ls = 1; ls <= 6; ++ls) {
@@ -212,8 +221,10 @@ int check_connectivity(int K, int S, const int* S2E, const int* E2S) {
       }
     }
   for (int s = 0; s < S; ++s)
-    if (seen[2 * s] != 1 || seen[2 * s + 1] != 1)
-      return fail("every side needs exactly one master and one slave (periodic, conforming mesh)");
+    if (seen[2 * s] != 1 || seen[2 * s + 1] != (s < nBC ? 0 : 1))
+      return fail("every inner side needs exactly one master and one slave, every boundary side (the first "
+                  "sides, SideToElem(2,:) = -1) one master");
+  if (nBCOut) *nBCOut = nBC;
   return 0;
 }
 
@@ -233,8 +244,9 @@ int alloc_owned(fxg_ctx* c, T** dst, size_t count) { return upload<T>(c, dst, nu
 
 // Allocations and tables shared by both creation paths. Geometry arrays are allocated but filled
 // by the caller.
-int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, const int* E2S) {
+int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, const int* E2S, int nBC) {
   c->p = *p;
+  c->nBC = nBC;
   HIPCHECK(hipGetDevice(&c->device));
   c->N = p->N; c->n = p->N + 1; c->nf = c->n * c->n; c->np = c->nf * c->n; c->K = K; c->S = S;
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -262,13 +274,14 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   if (alloc_owned(c, &c->w.Uf_m, 5 * face) || alloc_owned(c, &c->w.Uf_s, 5 * face) ||
       alloc_owned(c, &G, 12 * vol * gsz) || alloc_owned(c, &Gm, 12 * face * gsz) || alloc_owned(c, &Gs, 12 * face * gsz) ||
       alloc_owned(c, &c->w.Flux, 5 * face) || alloc_owned(c, &c->w.dtElem, 2 * (size_t)K) ||
-      alloc_owned(c, &c->w.errElem, (size_t)K) || alloc_owned(c, &c->w.dtOut, 2) || alloc_owned(c, &c->w.errOut, 1))
+      alloc_owned(c, &c->w.errElem, (size_t)K) || alloc_owned(c, &c->w.dtOut, 2) || alloc_owned(c, &c->w.errOut, 1) ||
+      alloc_owned(c, &c->w.bcLift, 4 * (size_t)c->nf * nBC))
     return 1;
   c->w.G = G; c->w.Gf_m = Gm; c->w.Gf_s = Gs;
   HIPCHECK(hipMemsetAsync(G, 0, 12 * vol * gsz, c->stream));
   // FXG_PIPELINE=classic keeps the four-kernel path for cases the fused one covers
   const char* pipe = std::getenv("FXG_PIPELINE");
-  c->fused = fxg::fused_supported(c->N, p->nodeType, p->splitDG == FXG_SPLITDG_PI, S) &&
+  c->fused = fxg::fused_supported(c->N, p->nodeType, p->splitDG == FXG_SPLITDG_PI, S) && nBC == 0 &&
              !(pipe && std::strcmp(pipe, "classic") == 0);
   if (c->fused) {
     char* FG;
@@ -307,6 +320,7 @@ void ctx_free(fxg_ctx* c) {
 // One right-hand side (st.mode 0) or LSERK stage. On the fused path `faces_fresh` says that
 // FU[cur] already holds the face states of U (written by the previous stage).
 int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStream_t s, bool faces_fresh = false) {
+  if (c->nBC > 0 && !c->bcSet) return fail("the mesh has boundary sides: set their conditions with fxg_set_boundaries");
   if (c->fused) {
     HIPCHECK(fxg::launch_rhs_fused(c->m, c->fw, c->gas, c->p.riemann, c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st,
                                    faces_fresh, c->cur, st.mode == 0 ? c->w.G : nullptr, s, c->timer));
@@ -404,14 +418,16 @@ int fxg_create(const fxg_params* p, const fxg_mesh* mesh, fxg_ctx** out) {
   *out = nullptr;
   if (check_params(p)) return 1;
   if (!mesh) return fail("mesh is NULL");
-  if (mesh->firstInnerSide != 1 || mesh->lastInnerSide != mesh->nSides)
-    return fail("only meshes without boundary or MPI sides are supported (firstInnerSide=1, lastInnerSide=nSides)");
-  if (check_connectivity(mesh->nElems, mesh->nSides, mesh->SideToElem, mesh->ElemToSide)) return 1;
+  int nBC = 0;
+  if (check_connectivity(mesh->nElems, mesh->nSides, mesh->SideToElem, mesh->ElemToSide, &nBC)) return 1;
+  if (mesh->firstInnerSide != nBC + 1 || mesh->lastInnerSide != mesh->nSides)
+    return fail("side ranges: boundary sides 1..nBCSides, then inner sides up to nSides (no mortar or MPI sides); "
+                "firstInnerSide must be nBCSides+1 and lastInnerSide nSides");
   if (!mesh->Metrics_fTilde || !mesh->Metrics_gTilde || !mesh->Metrics_hTilde || !mesh->sJ || !mesh->NormVec ||
       !mesh->TangVec1 || !mesh->TangVec2 || !mesh->SurfElem)
     return fail("a metric array is NULL");
   fxg_ctx* c = new fxg_ctx;
-  if (ctx_setup(c, p, mesh->nElems, mesh->nSides, mesh->SideToElem, mesh->ElemToSide)) { ctx_free(c); return 1; }
+  if (ctx_setup(c, p, mesh->nElems, mesh->nSides, mesh->SideToElem, mesh->ElemToSide, nBC)) { ctx_free(c); return 1; }
   const size_t vol = (size_t)c->np * c->K, face = (size_t)c->nf * c->S;
   struct { double* d; const double* h; size_t n; } cp[] = {
       {c->Mf, mesh->Metrics_fTilde, 3 * vol}, {c->Mg, mesh->Metrics_gTilde, 3 * vol}, {c->Mh, mesh->Metrics_hTilde, 3 * vol},
@@ -431,11 +447,12 @@ int fxg_create_from_nodes(const fxg_params* p, int nElems, int nSides, const int
   if (!out) return fail("ctx output pointer is NULL");
   *out = nullptr;
   if (check_params(p)) return 1;
-  if (check_connectivity(nElems, nSides, S2E, E2S)) return 1;
+  int nBC = 0;
+  if (check_connectivity(nElems, nSides, S2E, E2S, &nBC)) return 1;
   if (NGeo < 1 || NGeo > 8) return fail("NGeo must be between 1 and 8");
   if (!NodeCoords) return fail("NodeCoords is NULL");
   fxg_ctx* c = new fxg_ctx;
-  if (ctx_setup(c, p, nElems, nSides, S2E, E2S)) { ctx_free(c); return 1; }
+  if (ctx_setup(c, p, nElems, nSides, S2E, E2S, nBC)) { ctx_free(c); return 1; }
   const size_t ng = (size_t)(NGeo + 1) * (NGeo + 1) * (NGeo + 1);
   double* dNC = nullptr;
   if (upload(c, &dNC, NodeCoords, 3 * ng * nElems)) { ctx_free(c); return 1; }
@@ -457,6 +474,44 @@ int fxg_destroy(fxg_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   ctx_free(c);
+  return 0;
+}
+
+int fxg_set_boundaries(fxg_ctx* c, int nBCs, const int* BC, const int* BoundaryType, int nRefState,
+                       const double* RefStatePrim) {
+  if (!c) return fail("ctx is NULL");
+  if (c->nBC == 0) return 0;
+  if (!BC || !BoundaryType || nBCs < 1) return fail("BC and BoundaryType are required for a mesh with boundary sides");
+  if (nRefState < 0 || (nRefState > 0 && !RefStatePrim)) return fail("RefStatePrim is NULL");
+  std::vector<int> info(2 * (size_t)c->nBC);
+  for (int s = 0; s < c->nBC; ++s) {
+    const int iBC = BC[s];
+    if (iBC < 1 || iBC > nBCs) return fail("BC(SideID) out of range 1..nBCs");
+    const int type = BoundaryType[iBC - 1], state = BoundaryType[iBC - 1 + nBCs];
+    if (type != 2 && type != 3 && type != 4 && type != 9)
+      return fail("boundary type " + std::to_string(type) + " is not supported (2, 3, 4 and 9 are)");
+    if ((type == 2 || type == 4) && (state < 1 || state > nRefState))
+      return fail("boundary type " + std::to_string(type) + " needs a RefState index 1..nRefState as its state "
+                  "(state 0, an exact function, is not supported)");
+    if (type == 4 && !(RefStatePrim[6 * (state - 1) + 5] > 0.0))
+      return fail("an isothermal wall needs a positive wall temperature");
+    info[2 * s] = type;
+    info[2 * s + 1] = (type == 2 || type == 4) ? state - 1 : -1;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (!c->bcInfo && alloc_owned(c, &c->bcInfo, info.size())) return 1;
+  HIPCHECK(hipMemcpy(c->bcInfo, info.data(), info.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (c->refPrim) {
+    (void)hipFree(c->refPrim);
+    c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), (void*)c->refPrim), c->owned.end());
+    c->refPrim = nullptr;
+  }
+  if (upload(c, &c->refPrim, nRefState > 0 ? RefStatePrim : nullptr, 6 * (size_t)std::max(1, nRefState))) return 1;
+  c->m.nBC = c->nBC;
+  c->m.bcInfo = c->bcInfo;
+  c->m.refPrim = c->refPrim;
+  c->bcSet = true;
   return 0;
 }
 

@@ -8,6 +8,7 @@
 #include <hdf5.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -99,17 +100,83 @@ struct LocSide {
   int ind = 0;       // |SideInfo(SIDE_ID)|
   int flip = 0;      // 0 for the oriented (master) side
   int nbElem = 0;    // SideInfo(SIDE_nbElemID)
+  int bc = 0;        // SideInfo(SIDE_BCID)
+  bool boundary = false;  // a boundary side without a connection (BC type other than 0, 1, 100)
   int sideID = -1;   // assigned by setLocalSideIDs
   int conn = -1;     // index (6*elem + locSide) of the connected side
 };
 
 struct MeshData {
-  int nElems = 0, NGeo = 0, nSides = 0;
+  int nElems = 0, NGeo = 0, nSides = 0, nBCSides = 0;
   std::vector<LocSide> sides;  // 6 per element
   std::vector<double> nodes;   // (3,0:NGeo,0:NGeo,0:NGeo,nElems)
+  std::vector<std::string> bcNames;
+  std::vector<int> bcType;     // BCType(4,nBCs) after the parameter file's BoundaryType
 };
 
-int load_mesh(const char* path, MeshData& md, bool withNodes) {
+// BoundaryName / BoundaryType pairs of the parameter file (ReadBCs, mesh_readin.f90:92-100).
+struct UserBCs {
+  int n = 0;
+  const char* const* names = nullptr;
+  const int* types = nullptr;  // (type, state) per name
+};
+
+std::string lower_trim(std::string v) {
+  while (!v.empty() && (v.back() == ' ' || v.back() == '\0')) v.pop_back();
+  for (char& ch : v) ch = (char)std::tolower((unsigned char)ch);
+  return v;
+}
+
+// BCNames(nBCs) (fixed-length strings, blank padded).
+int read_bc_names(hid_t file, size_t nBCs, std::vector<std::string>& out) {
+  H d(H5Dopen2(file, "BCNames", H5P_DEFAULT), H5Dclose);
+  if (d.id < 0) return fail("dataset BCNames missing");
+  H t(H5Dget_type(d.id), H5Tclose);
+  if (H5Tget_class(t.id) != H5T_STRING || H5Tis_variable_str(t.id) > 0)
+    return fail("BCNames is not a fixed-length string dataset");
+  H s(H5Dget_space(d.id), H5Sclose);
+  if (H5Sget_simple_extent_npoints(s.id) != (hssize_t)nBCs) return fail("BCNames and BCType differ in length");
+  const size_t len = H5Tget_size(t.id);
+  std::vector<char> buf(len * nBCs + 1, '\0');
+  if (H5Dread(d.id, t.id, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf.data()) < 0) return fail("cannot read BCNames");
+  out.clear();
+  for (size_t i = 0; i < nBCs; ++i) {
+    std::string v(&buf[i * len], len);
+    const size_t z = v.find('\0');
+    if (z != std::string::npos) v.resize(z);
+    while (!v.empty() && v.back() == ' ') v.pop_back();
+    out.push_back(v);
+  }
+  return 0;
+}
+
+// ReadBCs (mesh_readin.f90:113-164): a user boundary replaces the type and state of the file's
+// boundary of the same name (case-insensitive); every user name must exist; a non-periodic
+// boundary cannot become periodic.
+int apply_user_bcs(const UserBCs& u, const std::vector<std::string>& names, std::vector<int>& bct) {
+  std::vector<int> map(names.size(), -1);
+  for (int k = 0; k < u.n; ++k) {
+    if (!u.names || !u.names[k] || !u.types) return fail("NULL user boundary name or type");
+    bool found = false;
+    for (size_t i = 0; i < names.size(); ++i)
+      if (lower_trim(names[i]) == lower_trim(u.names[k])) {
+        if (map[i] >= 0) return fail("boundary " + names[i] + " is redefined multiple times");
+        map[i] = k;
+        found = true;
+      }
+    if (!found) return fail(std::string("boundary condition not found in the mesh file: ") + u.names[k]);
+  }
+  for (size_t i = 0; i < names.size(); ++i) {
+    if (map[i] < 0) continue;
+    const int type = u.types[2 * map[i]], state = u.types[2 * map[i] + 1];
+    if (type == 1 && bct[4 * i] != 1) return fail("remapping non-periodic to periodic boundaries is not possible");
+    bct[4 * i] = type;
+    bct[4 * i + 2] = state;
+  }
+  return 0;
+}
+
+int load_mesh(const char* path, MeshData& md, bool withNodes, bool allowBC = false, const UserBCs& user = UserBCs{}) {
   Quiet q;
   H f(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
   if (f.id < 0) return fail(std::string("cannot open mesh file ") + path);
@@ -123,6 +190,10 @@ int load_mesh(const char* path, MeshData& md, bool withNodes) {
   std::vector<int> bct;
   size_t nBCs = 0;
   if (H5Lexists(f.id, "BCType", H5P_DEFAULT) > 0 && read_int_2d(f.id, "BCType", 4, bct, &nBCs)) return 1;
+  md.bcNames.clear();
+  if (nBCs > 0 && read_bc_names(f.id, nBCs, md.bcNames)) return 1;
+  if (user.n > 0 && apply_user_bcs(user, md.bcNames, bct)) return 1;
+  md.bcType = bct;
   std::vector<int> ei, si;
   size_t rowsE = 0, rowsS = 0;
   if (read_int_2d(f.id, "ElemInfo", 6, ei, &rowsE) || read_int_2d(f.id, "SideInfo", 5, si, &rowsS)) return 1;
@@ -147,16 +218,21 @@ int load_mesh(const char* path, MeshData& md, bool withNodes) {
       a.flip = S[1] > 0 ? 0 : S[3] % 10;
       if (a.flip < 0 || a.flip > 4) return fail("side flip out of range");
       a.nbElem = S[2];
-      const int bc = S[4];
-      if (bc != 0) {
-        if (bc < 0 || (size_t)bc > nBCs) return fail("SideInfo BCID out of range");
-        const int type = bct[(size_t)4 * (bc - 1)];
-        // only sides that FLEXI connects like inner sides (mesh_readin.f90:413-422)
-        if (type != 0 && type != 1 && type != 100)
-          return fail("boundary sides are not supported (BC type " + std::to_string(type) +
-                      "): only inner and periodic sides");
+      a.bc = S[4];
+      if (a.bc != 0) {
+        if (a.bc < 0 || (size_t)a.bc > nBCs) return fail("SideInfo BCID out of range");
+        const int type = bct[(size_t)4 * (a.bc - 1)];
+        // sides of other types are boundary sides: no connection, flip 0 (mesh_readin.f90:413-422)
+        if (type != 0 && type != 1 && type != 100) {
+          if (!allowBC)
+            return fail("boundary sides are not supported (BC type " + std::to_string(type) +
+                        "): only inner and periodic sides");
+          a.boundary = true;
+          a.flip = 0;
+          continue;
+        }
       }
-      if (a.nbElem == 0) return fail("side without a neighbour element (non-periodic boundary)");
+      if (a.nbElem == 0) return fail("side without a neighbour element and without a boundary condition");
       if (a.nbElem > K) return fail("neighbour element index out of range");
     }
   }
@@ -164,7 +240,7 @@ int load_mesh(const char* path, MeshData& md, bool withNodes) {
   for (int e = 0; e < K; ++e)
     for (int ls = 0; ls < 6; ++ls) {
       LocSide& a = md.sides[(size_t)6 * e + ls];
-      if (a.conn >= 0) continue;
+      if (a.conn >= 0 || a.boundary) continue;
       const int nb = a.nbElem - 1;
       for (int ms = 0; ms < 6; ++ms) {
         const int j = 6 * nb + ms;
@@ -177,17 +253,25 @@ int load_mesh(const char* path, MeshData& md, bool withNodes) {
       }
       if (a.conn < 0) return fail("side " + std::to_string(a.ind) + " has no partner in its neighbour element");
     }
-  // side IDs in element / local-side order (prepare_mesh.f90:199-243: all sides are inner)
-  int next = 0;
+  // side IDs in element / local-side order (prepare_mesh.f90:196-243): boundary sides first
+  // (1..nBCSides), then the inner sides
+  int nBC = 0;
+  for (const LocSide& a : md.sides) nBC += a.boundary ? 1 : 0;
+  int nextBC = 0, next = nBC;
   for (int e = 0; e < K; ++e)
     for (int ls = 0; ls < 6; ++ls) {
       LocSide& a = md.sides[(size_t)6 * e + ls];
       if (a.sideID >= 0) continue;
+      if (a.boundary) {
+        a.sideID = ++nextBC;
+        continue;
+      }
       LocSide& b = md.sides[a.conn];
       if ((a.flip == 0) == (b.flip == 0)) return fail("side " + std::to_string(a.ind) + " needs one master and one slave");
       a.sideID = b.sideID = ++next;
     }
   md.nSides = next;
+  md.nBCSides = nBC;
   if (withNodes) {
     H d(H5Dopen2(f.id, "NodeCoords", H5P_DEFAULT), H5Dclose);
     if (d.id < 0) return fail("dataset NodeCoords missing");
@@ -210,6 +294,29 @@ int load_mesh(const char* path, MeshData& md, bool withNodes) {
     }
   }
   return 0;
+}
+
+// fillMeshInfo (prepare_mesh.f90:690-716): SideToElem initialised to -1 as in InitMesh
+// (mesh.f90:295-297), the flip-0 element of a side is its master, BC(SideID) for boundary sides.
+void fill_mesh_info(const MeshData& md, int* SideToElem, int* ElemToSide, double* NodeCoords, int* BC) {
+  for (size_t i = 0; i < (size_t)5 * md.nSides; ++i) SideToElem[i] = -1;
+  for (int e = 0; e < md.nElems; ++e)
+    for (int ls = 0; ls < 6; ++ls) {
+      const LocSide& a = md.sides[(size_t)6 * e + ls];
+      ElemToSide[2 * ls + 12 * (size_t)e] = a.sideID;
+      ElemToSide[2 * ls + 1 + 12 * (size_t)e] = a.flip;
+      int* s2e = SideToElem + (size_t)5 * (a.sideID - 1);
+      if (a.flip == 0) {
+        s2e[0] = e + 1;
+        s2e[2] = ls + 1;
+      } else {
+        s2e[1] = e + 1;
+        s2e[3] = ls + 1;
+        s2e[4] = a.flip;
+      }
+      if (BC && a.boundary) BC[a.sideID - 1] = a.bc;
+    }
+  std::memcpy(NodeCoords, md.nodes.data(), md.nodes.size() * sizeof(double));
 }
 
 hid_t str_type(size_t len) {
@@ -266,24 +373,41 @@ int fxg_io_read_mesh(const char* path, int nElems, int nSides, int NGeo, int* Si
   if (load_mesh(path, md, true)) return 1;
   if (md.nElems != nElems || md.nSides != nSides || md.NGeo != NGeo)
     return fail("array sizes differ from the mesh file (call fxg_io_mesh_size first)");
-  // fillMeshInfo (prepare_mesh.f90:690-716): SideToElem initialised to -1 as in InitMesh
-  for (size_t i = 0; i < (size_t)5 * nSides; ++i) SideToElem[i] = -1;
-  for (int e = 0; e < nElems; ++e)
-    for (int ls = 0; ls < 6; ++ls) {
-      const LocSide& a = md.sides[(size_t)6 * e + ls];
-      ElemToSide[2 * ls + 12 * (size_t)e] = a.sideID;
-      ElemToSide[2 * ls + 1 + 12 * (size_t)e] = a.flip;
-      int* s2e = SideToElem + (size_t)5 * (a.sideID - 1);
-      if (a.flip == 0) {
-        s2e[0] = e + 1;
-        s2e[2] = ls + 1;
-      } else {
-        s2e[1] = e + 1;
-        s2e[3] = ls + 1;
-        s2e[4] = a.flip;
-      }
-    }
-  std::memcpy(NodeCoords, md.nodes.data(), md.nodes.size() * sizeof(double));
+  fill_mesh_info(md, SideToElem, ElemToSide, NodeCoords, nullptr);
+  return 0;
+}
+
+int fxg_io_mesh_size_bc(const char* path, int nUser, const char* const* userNames, const int* userTypes, int* nElems,
+                        int* nSides, int* NGeo, int* nBCSides, int* nBCs) {
+  if (!path || !nElems || !nSides || !NGeo || !nBCSides || !nBCs || nUser < 0) return fail("invalid argument");
+  MeshData md;
+  if (load_mesh(path, md, false, true, UserBCs{nUser, userNames, userTypes})) return 1;
+  *nElems = md.nElems;
+  *nSides = md.nSides;
+  *NGeo = md.NGeo;
+  *nBCSides = md.nBCSides;
+  *nBCs = (int)md.bcNames.size();
+  return 0;
+}
+
+int fxg_io_read_mesh_bc(const char* path, int nUser, const char* const* userNames, const int* userTypes, int nElems,
+                        int nSides, int NGeo, int nBCSides, int nBCs, int* SideToElem, int* ElemToSide,
+                        double* NodeCoords, int* BC, int* BoundaryType, char* BCNames, int nameLen) {
+  if (!path || !SideToElem || !ElemToSide || !NodeCoords || nUser < 0) return fail("invalid argument");
+  if ((nBCSides > 0 && !BC) || (nBCs > 0 && !BoundaryType)) return fail("BC or BoundaryType is NULL");
+  MeshData md;
+  if (load_mesh(path, md, true, true, UserBCs{nUser, userNames, userTypes})) return 1;
+  if (md.nElems != nElems || md.nSides != nSides || md.NGeo != NGeo || md.nBCSides != nBCSides ||
+      (int)md.bcNames.size() != nBCs)
+    return fail("array sizes differ from the mesh file (call fxg_io_mesh_size_bc first)");
+  fill_mesh_info(md, SideToElem, ElemToSide, NodeCoords, BC);
+  // BoundaryType(nBCs,3) = (type, state, alpha), mesh_readin.f90:168-173
+  for (int i = 0; i < nBCs; ++i) {
+    BoundaryType[i] = md.bcType[(size_t)4 * i];
+    BoundaryType[i + nBCs] = md.bcType[(size_t)4 * i + 2];
+    BoundaryType[i + 2 * nBCs] = md.bcType[(size_t)4 * i + 3];
+    if (BCNames && nameLen > 0) std::snprintf(BCNames + (size_t)i * nameLen, (size_t)nameLen, "%s", md.bcNames[i].c_str());
+  }
   return 0;
 }
 

@@ -41,6 +41,11 @@ struct DevMesh {
   // terms at the face node (metrics computed by the library), 0 = the side's NormVec/SurfElem as
   // supplied by the host, shared by master and slave as FLEXI does
   int ownFaceMetric;
+  // boundary sides 0..nBC-1 (FLEXI numbers them first): per side (BC type, RefState index or -1),
+  // and RefStatePrim(6,nRefState)
+  int nBC = 0;
+  const int* bcInfo = nullptr;
+  const double* refPrim = nullptr;
 };
 
 // Work arrays of one right-hand side evaluation. Face arrays of U are FP64 (U(5,p,q,side));
@@ -52,6 +57,7 @@ struct DevWork {
   void* Gf_m;      // face gradients (12,0:N,0:N,nSides)
   void* Gf_s;
   double* Flux;    // Flux_master(5,0:N,0:N,nSides); Flux_slave equals it (fillflux.f90:199)
+  double* bcLift;  // lifting flux of the boundary sides before SurfElem, (4,0:N,0:N,nBC)
   double* dtElem;  // (nElems)
   int* errElem;    // (nElems)
   double* dtOut;   // [2]: convective and viscous minima

@@ -6,6 +6,7 @@
 // Side kernels run one lane per side point. No atomics: every element gathers the fluxes of its
 // own six sides, so results do not depend on scheduling.
 #include "fxg_kernels.h"
+#include "fxg_bc.h"
 
 namespace fxg {
 
@@ -99,8 +100,8 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_prolong_u(const double* __restri
 template <int N, bool GL, class RF>
 __global__ __launch_bounds__(Dim<N>::NT) void k_lift(const double* __restrict__ U, DevMesh m,
                                                    const double* __restrict__ Um, const double* __restrict__ Us,
-                                                   Gas gas, RF* __restrict__ G, RF* __restrict__ Gm,
-                                                   RF* __restrict__ Gs) {
+                                                   const double* __restrict__ bcLift, Gas gas, RF* __restrict__ G,
+                                                   RF* __restrict__ Gm, RF* __restrict__ Gs) {
   constexpr int n = Dim<N>::n, nf = Dim<N>::nf, np = Dim<N>::np, NT = Dim<N>::NT;
   __shared__ double sA[12 * np];
   __shared__ double sB[3 * n * n + 4 * n];
@@ -162,19 +163,25 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_lift(const double* __restrict__ 
       const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
       const int* pq2 = m.v2s2 + 2 * ((a + n * b) + nf * (flip + 5 * (ls - 1)));
       const size_t fp = (size_t)(pq2[0] + n * pq2[1]) + (size_t)nf * side;
-      const Prim pm = cons_to_prim(Um + 5 * fp, gas);
-      const Prim ps = cons_to_prim(Us + 5 * fp, gas);
       const RF se = (RF)m.SurfElem[fp];
       const RF nv[3] = {(RF)m.NormVec[3 * fp], (RF)m.NormVec[3 * fp + 1], (RF)m.NormVec[3 * fp + 2]};
       const RF lh = GL ? (RF)(is_minus(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N])
                        : (RF)(is_minus(ls) ? sB[B.L_HatMinus() + l] : sB[B.L_HatPlus() + l]);
-      const RF dq[4] = {(RF)ps.u - (RF)pm.u, (RF)ps.v - (RF)pm.v, (RF)ps.w - (RF)pm.w, (RF)ps.T - (RF)pm.T};
+      RF f[4];
+      if (side < m.nBC) {  // boundary side: Lifting_GetBoundaryFlux (k_bc_lift) times SurfElem
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const RF f = RF(0.5) * dq[v] * se;
+        for (int v = 0; v < 4; ++v) f[v] = (RF)bcLift[4 * fp + v] * se;
+      } else {
+        const Prim pm = cons_to_prim(Um + 5 * fp, gas);
+        const Prim ps = cons_to_prim(Us + 5 * fp, gas);
+        const RF dq[4] = {(RF)ps.u - (RF)pm.u, (RF)ps.v - (RF)pm.v, (RF)ps.w - (RF)pm.w, (RF)ps.T - (RF)pm.T};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) g[4 * d + v] = g[4 * d + v] + f * nv[d] * lh;
+        for (int v = 0; v < 4; ++v) f[v] = RF(0.5) * dq[v] * se;
       }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[4 * d + v] = g[4 * d + v] + f[v] * nv[d] * lh;
     }
     const RF sj = (RF)m.sJ[gid];
 #pragma unroll
@@ -224,8 +231,8 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_lift(const double* __restrict__ 
 template <class RF, bool SPLIT>
 __global__ __launch_bounds__(256) void k_flux(DevMesh m, const double* __restrict__ Um, const double* __restrict__ Us,
                                             const RF* __restrict__ Gm, const RF* __restrict__ Gs, Gas gas,
-                                            int solver, double* __restrict__ Flux, int nPts) {
-  const int fp = blockIdx.x * blockDim.x + threadIdx.x;
+                                            int solver, double* __restrict__ Flux, int first, int nPts) {
+  const int fp = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (fp >= nPts) return;
   const GasT<RF> g(gas);
   const Prim pl = cons_to_prim(Um + 5 * (size_t)fp, gas);
@@ -262,6 +269,100 @@ __global__ __launch_bounds__(256) void k_flux(DevMesh m, const double* __restric
   const double se = m.SurfElem[fp];
 #pragma unroll
   for (int v = 0; v < 5; ++v) Flux[5 * (size_t)fp + v] = (double)Fo[v] * se;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Boundary sides 0..nBC-1 (FLEXI numbers them first). One lane per boundary side point; the inner
+// state comes from the prolonged conservative variables as GetPrimitiveStateSurface does.
+__device__ __forceinline__ int bc_point(const DevMesh& m, const double* __restrict__ Um, const Gas& gas, int fp,
+                                        double* Pm, double* Pb, double* nv, double* t1, double* t2) {
+  const Prim q = cons_to_prim(Um + 5 * (size_t)fp, gas);
+  Pm[0] = q.rho; Pm[1] = q.u; Pm[2] = q.v; Pm[3] = q.w; Pm[4] = q.p; Pm[5] = q.T;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    nv[c] = m.NormVec[3 * (size_t)fp + c];
+    t1[c] = m.TangVec1[3 * (size_t)fp + c];
+    t2[c] = m.TangVec2[3 * (size_t)fp + c];
+  }
+  const int side = fp / ((m.N + 1) * (m.N + 1));
+  const int type = m.bcInfo[2 * side], st = m.bcInfo[2 * side + 1];
+  boundary_state(gas, type, st >= 0 ? m.refPrim + 6 * st : nullptr, Pm, nv, t1, t2, Pb);
+  return type;
+}
+
+// Lifting flux of the boundary sides (Lifting_FillFlux_BC, lifting_fillflux.t90:133-180) before
+// SurfElem, for k_lift.
+__global__ __launch_bounds__(256) void k_bc_lift(DevMesh m, const double* __restrict__ Um, Gas gas,
+                                               double* __restrict__ bcLift, int nPts) {
+  const int fp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fp >= nPts) return;
+  double Pm[6], Pb[6], nv[3], t1[3], t2[3], F[4];
+  const int type = bc_point(m, Um, gas, fp, Pm, Pb, nv, t1, t2);
+  boundary_lifting_flux(type, Pm, Pb, F);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) bcLift[4 * (size_t)fp + v] = F[v];
+}
+
+// GetBoundaryFlux (getboundaryflux.f90:480-789) times SurfElem (fillflux.f90:192-199): Riemann
+// solver against the boundary state plus ViscousFlux for Dirichlet sides; for walls the Euler
+// flux (0, p n, 0) plus the viscous flux of the wall state with the inner gradients (without
+// their normal part at a slip wall; no heat flux at an adiabatic wall).
+template <class RF>
+__global__ __launch_bounds__(256) void k_bc_flux(DevMesh m, const double* __restrict__ Um, const RF* __restrict__ Gm,
+                                               Gas gas, int solver, int split, double* __restrict__ Flux, int nPts) {
+  const int fp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fp >= nPts) return;
+  double Pm[6], Pb[6], nv[3], t1[3], t2[3], F[5], q[12];
+  const int type = bc_point(m, Um, gas, fp, Pm, Pb, nv, t1, t2);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) q[c] = (double)Gm[12 * (size_t)fp + c];
+  const GasT<double> g(gas);
+  if (type == BC_DIRICHLET) {
+    // both states' conservative variables from the primitive ones (PrimToCons), as GetBoundaryFlux
+    const double Em = gas.sKappaM1 * Pm[4] + 0.5 * ((Pm[1] * Pm[0]) * Pm[1] + (Pm[2] * Pm[0]) * Pm[2] + (Pm[3] * Pm[0]) * Pm[3]);
+    const double Eb = gas.sKappaM1 * Pb[4] + 0.5 * ((Pb[1] * Pb[0]) * Pb[1] + (Pb[2] * Pb[0]) * Pb[2] + (Pb[3] * Pb[0]) * Pb[3]);
+    const RotState<double> L = rotate<double>(Pm[0], Em, Pm[4], Pm[1], Pm[2], Pm[3], nv, t1, t2);
+    const RotState<double> R = rotate<double>(Pb[0], Eb, Pb[4], Pb[1], Pb[2], Pb[3], nv, t1, t2);
+    double Fr[5];
+    riemann_rot<double>(g, solver, split != 0, L, R, Fr);
+    F[0] = Fr[0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) F[1 + c] = nv[c] * Fr[1] + t1[c] * Fr[2] + t2[c] * Fr[3];
+    F[4] = Fr[4];
+    double fxl[4], fyl[4], fzl[4], fxr[4], fyr[4], fzr[4];
+    diff_flux<double>(g, Pm[1], Pm[2], Pm[3], q, q + 4, q + 8, fxl, fyl, fzl);
+    diff_flux<double>(g, Pb[1], Pb[2], Pb[3], q, q + 4, q + 8, fxr, fyr, fzr);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      F[1 + k] = F[1 + k] + 0.5 * (nv[0] * (fxl[k] + fxr[k]) + nv[1] * (fyl[k] + fyr[k]) + nv[2] * (fzl[k] + fzr[k]));
+  } else {
+    F[0] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[1 + d] = Pb[4] * nv[d];
+    F[4] = 0.0;
+    if (type == BC_WALL_SLIP) {  // (I - n n^T) applied to every lifted gradient
+      double M[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) M[a][b] = a == b ? 1.0 - nv[a] * nv[a] : -nv[a] * nv[b];
+      double r[12];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[4 * a + v] = M[a][0] * q[v] + M[a][1] * q[4 + v] + M[a][2] * q[8 + v];
+#pragma unroll
+      for (int c = 0; c < 12; ++c) q[c] = r[c];
+    }
+    double fx[4], fy[4], fz[4];
+    diff_flux<double>(g, Pb[1], Pb[2], Pb[3], q, q + 4, q + 8, fx, fy, fz);
+    if (type == BC_WALL_ADIABATIC) fx[3] = fy[3] = fz[3] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[1 + k] = F[1 + k] + nv[0] * fx[k] + nv[1] * fy[k] + nv[2] * fz[k];
+  }
+  const double se = m.SurfElem[fp];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Flux[5 * (size_t)fp + v] = F[v] * se;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -584,20 +685,28 @@ static hipError_t rhs_n(const DevMesh& m, const DevWork& w, const Gas& gas, int 
     TimedScope ts(tm, K_PROLONG, s);
     hipLaunchKernelGGL((k_prolong_u<N, GL>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s);
   }
+  const int nBCPts = m.nBC * Dim<N>::nf;
   {
     TimedScope ts(tm, K_LIFT, s);
-    hipLaunchKernelGGL((k_lift<N, GL, RF>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s, gas, (RF*)w.G,
+    if (nBCPts > 0)
+      hipLaunchKernelGGL(k_bc_lift, dim3((nBCPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, gas, w.bcLift, nBCPts);
+    hipLaunchKernelGGL((k_lift<N, GL, RF>), dim3(K), dim3(NT), 0, s, U, m, w.Uf_m, w.Uf_s, w.bcLift, gas, (RF*)w.G,
                        (RF*)w.Gf_m, (RF*)w.Gf_s);
   }
-  const int nPts = m.nSides * Dim<N>::nf;
+  const int nPts = m.nSides * Dim<N>::nf, nIn = nPts - nBCPts;
   {
     TimedScope ts(tm, K_FLUX, s);
-    if (split)
-      hipLaunchKernelGGL((k_flux<RF, true>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
-                         (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
-    else
-      hipLaunchKernelGGL((k_flux<RF, false>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
-                         (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nPts);
+    if (nIn > 0) {
+      if (split)
+        hipLaunchKernelGGL((k_flux<RF, true>), dim3((nIn + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
+                           (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nBCPts, nPts);
+      else
+        hipLaunchKernelGGL((k_flux<RF, false>), dim3((nIn + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
+                           (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nBCPts, nPts);
+    }
+    if (nBCPts > 0)
+      hipLaunchKernelGGL((k_bc_flux<RF>), dim3((nBCPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, (const RF*)w.Gf_m,
+                         gas, riemann, split, w.Flux, nBCPts);
   }
   {
     TimedScope ts(tm, K_VOLINT, s);

@@ -67,6 +67,14 @@ INTERFACE
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR),VALUE :: ctx
   END FUNCTION
+  INTEGER(C_INT) FUNCTION fxg_set_boundaries(ctx,nBCs,BC,BoundaryType,nRefState,RefStatePrim) &
+      BIND(C,NAME='fxg_set_boundaries')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR),VALUE         :: ctx
+    INTEGER(C_INT),VALUE      :: nBCs, nRefState
+    INTEGER(C_INT),INTENT(IN) :: BC(*), BoundaryType(*)
+    REAL(C_DOUBLE),INTENT(IN) :: RefStatePrim(*)
+  END FUNCTION
   INTEGER(C_INT) FUNCTION fxg_set_state(ctx,U) BIND(C,NAME='fxg_set_state')
     IMPORT :: C_INT, C_DOUBLE, C_PTR
     TYPE(C_PTR),VALUE         :: ctx

@@ -38,8 +38,11 @@ def lib() -> C.CDLL:
         L.fxg_io_state_size.argtypes = [cp, ip, ip, ip]
         L.fxg_io_read_state.argtypes = [cp, C.c_int, C.c_int, C.c_int, dp, dp, cp, C.c_int, cp, C.c_int]
         L.fxg_io_write_state.argtypes = [cp, cp, cp, cp, C.c_int, C.c_int, C.c_double, dp]
+        cpp = C.POINTER(C.c_char_p)
+        L.fxg_io_mesh_size_bc.argtypes = [cp, C.c_int, cpp, ip, ip, ip, ip, ip, ip]
+        L.fxg_io_read_mesh_bc.argtypes = [cp, C.c_int, cpp, ip] + [C.c_int] * 5 + [ip, ip, dp, ip, ip, cp, C.c_int]
         for f in ("fxg_io_mesh_size", "fxg_io_read_mesh", "fxg_io_state_size", "fxg_io_read_state",
-                  "fxg_io_write_state"):
+                  "fxg_io_write_state", "fxg_io_mesh_size_bc", "fxg_io_read_mesh_bc"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -69,6 +72,38 @@ def read_mesh(path: str) -> Mesh:
     hi = nodes.reshape(3, -1).max(axis=1)
     return Mesh(nElems=K, nSides=S, NGeo=NGeo, NodeCoords=nodes, ElemToSide=e2s, SideToElem=s2e,
                 shape=None, box_lo=tuple(float(x) for x in lo), box_len=tuple(float(x) for x in hi - lo))
+
+
+def read_mesh_bc(path: str, boundaries=None) -> Mesh:
+    """A HOPR mesh file with boundary sides (walls, inflow, ...): as ``read_mesh`` plus
+    ``nBCSides``, ``BC`` and ``BoundaryType``/``BCNames`` (ReadBCs + ReadMesh, mesh_readin.f90:70-470).
+    ``boundaries``: the parameter file's BoundaryName/BoundaryType pairs, [(name, (type, state)), ...],
+    which override the file's boundary of the same name."""
+    L = lib()
+    boundaries = list(boundaries or [])
+    nu = len(boundaries)
+    names = (C.c_char_p * max(1, nu))(*[n.encode() for n, _ in boundaries])
+    types = np.asfortranarray(np.array([list(t)[:2] for _, t in boundaries] or [[0, 0]], dtype=np.int32).T)
+    sz = [C.c_int() for _ in range(5)]
+    _check(L.fxg_io_mesh_size_bc(path.encode(), nu, names, _ptr(types, C.c_int), *[C.byref(x) for x in sz]))
+    K, S, NGeo, nbs, nbc = [x.value for x in sz]
+    s2e = np.zeros((5, S), dtype=np.int32, order="F")
+    e2s = np.zeros((2, 6, K), dtype=np.int32, order="F")
+    nodes = np.zeros((3, NGeo + 1, NGeo + 1, NGeo + 1, K), dtype=np.float64, order="F")
+    bc = np.zeros(max(1, nbs), dtype=np.int32)
+    btype = np.zeros((max(1, nbc), 3), dtype=np.int32, order="F")
+    nl = 256
+    bnames = C.create_string_buffer(nl * max(1, nbc))
+    _check(L.fxg_io_read_mesh_bc(path.encode(), nu, names, _ptr(types, C.c_int), K, S, NGeo, nbs, nbc,
+                                 _ptr(s2e, C.c_int), _ptr(e2s, C.c_int), _ptr(nodes, C.c_double), _ptr(bc, C.c_int),
+                                 _ptr(btype, C.c_int), bnames, nl))
+    lo = nodes.reshape(3, -1).min(axis=1)
+    hi = nodes.reshape(3, -1).max(axis=1)
+    raw = bnames.raw
+    return Mesh(nElems=K, nSides=S, NGeo=NGeo, NodeCoords=nodes, ElemToSide=e2s, SideToElem=s2e, shape=None,
+                box_lo=tuple(float(x) for x in lo), box_len=tuple(float(x) for x in hi - lo), nBCSides=nbs,
+                BC=bc[:nbs].copy(), BoundaryType=np.asfortranarray(btype[:nbc]),
+                BCNames=[raw[i * nl:(i + 1) * nl].split(b"\0", 1)[0].decode() for i in range(nbc)])
 
 
 def write_state(path: str, U: np.ndarray, N: int, node_type: str, time: float, mesh_file: str,

@@ -94,10 +94,15 @@ class Mesh:
     shape: Optional[tuple[int, int, int]] = None       # elements per direction of a generated box
     box_lo: Optional[tuple[float, float, float]] = None  # bounding box of the nodes
     box_len: Optional[tuple[float, float, float]] = None
+    # boundary sides 1..nBCSides (numbered first, prepare_mesh.f90:196-222)
+    nBCSides: int = 0
+    BC: Optional[np.ndarray] = None            # (nBCSides) int32, 1-based boundary index of each side
+    BoundaryType: Optional[np.ndarray] = None  # (nBCs,3) int32: type, state, alpha
+    BCNames: Optional[list] = None
 
     @property
     def firstInnerSide(self) -> int:
-        return 1
+        return self.nBCSides + 1
 
     @property
     def lastInnerSide(self) -> int:

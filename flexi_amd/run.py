@@ -14,9 +14,9 @@ Keys read: ProjectName, MeshFile, N, Riemann (LF, Roe, RoeEntropyFix), SplitDG (
 form), IniExactFunc with IniRefState/RefState (the Taylor-Green vortex of the TGV testcase, or a
 uniform RefState for IniExactFunc = 1), kappa, R, Pr, mu0, TStart, TEnd, CFLScale, DFLScale,
 Analyze_dt, nWriteData, maxIter, TimeDiscMethod, nCalcTimeStepMax (only 1), MachNumber and
-IniConstDens (T only) of the TGV testcase. The node type is a
-compile-time choice in FLEXI (FLEXI_NODETYPE); here it is ``--node-type``. Meshes must be
-periodic (the sides the kernels support); other keys are reported and ignored.
+IniConstDens (T only) of the TGV testcase, BoundaryName/BoundaryType (boundary types 2 with a
+RefState, 3, 4, 9). The node type is a compile-time choice in FLEXI (FLEXI_NODETYPE); here it is
+``--node-type``. Other keys are reported and ignored.
 """
 from __future__ import annotations
 
@@ -28,13 +28,13 @@ import time
 
 import numpy as np
 
-from . import io, testcase
+from . import equation, io, testcase
 from .solver import DGSolver
 from .timedisc import SCHEMES
 
 USED = {"projectname", "meshfile", "n", "riemann", "splitdg", "iniexactfunc", "inirefstate", "refstate", "kappa",
         "r", "pr", "mu0", "tstart", "tend", "cflscale", "dflscale", "analyze_dt", "nwritedata", "maxiter",
-        "timediscmethod", "ncalctimestepmax", "machnumber", "iniconstdens"}
+        "timediscmethod", "ncalctimestepmax", "machnumber", "iniconstdens", "boundaryname", "boundarytype"}
 
 
 def parse_parameter_file(path: str) -> dict:
@@ -123,25 +123,30 @@ class Case:
         if get("iniconstdens", "T").upper().strip(".") not in ("T", "TRUE"):
             raise ValueError("IniConstDens = F (constant temperature TGV) is not supported")
         self.ref_states = [_array(v) for v in prm.get("refstate", [])]
+        # BoundaryName / BoundaryType pairs (ReadBCs, mesh_readin.f90:92-100)
+        names, types = prm.get("boundaryname", []), prm.get("boundarytype", [])
+        if len(names) != len(types):
+            raise ValueError("every BoundaryName needs a BoundaryType")
+        self.boundaries = [(nm, tuple(int(x) for x in _array(tp))[:2]) for nm, tp in zip(names, types)]
         self.ini_ref = get("inirefstate", 1, int)
         self.ignored = sorted(k for k in prm if k not in USED)
 
     def initial_state(self, xgp: np.ndarray) -> np.ndarray:
-        if self.ini_exact == 1:  # uniform RefState (rho, u, v, w, p)
-            rho, u, v, w, p = self.ref_states[self.ini_ref - 1][:5]
-            U = np.empty((5,) + xgp.shape[1:], order="F")
-            U[0], U[1], U[2], U[3] = rho, rho * u, rho * v, rho * w
-            U[4] = p / (self.kappa - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
-            return U
+        if self.ini_exact == 1:  # RefStateCons(IniRefState) everywhere (exactfunc.f90:258-259)
+            prim = equation.ref_state_prim(self.ref_states, self.R)[:, self.ini_ref - 1]
+            cons = equation.prim_to_cons(prim, self.kappa)
+            return np.asfortranarray(np.broadcast_to(cons.reshape((5,) + (1,) * (xgp.ndim - 1)),
+                                                     (5,) + xgp.shape[1:]))
         # the Taylor-Green vortex of the TGV testcase (testcase.f90:176-219)
         return testcase.tgv(xgp, kappa=self.kappa, R=self.R, Ma=self.mach)
 
 
 def run(case: Case, out_dir: str, precision: str = "fp64", restart: str | None = None, log=print) -> dict:
-    mesh = io.read_mesh(case.mesh_path)
+    mesh = io.read_mesh_bc(case.mesh_path, case.boundaries)
+    ref_prim = equation.ref_state_prim(case.ref_states, case.R) if case.ref_states else None
     s = DGSolver(mesh, case.N, node_type=case.node_type, split_dg=case.split, riemann=case.riemann,
                  kappa=case.kappa, R=case.R, Pr=case.Pr, mu0=case.mu0, precision=precision, scheme=case.scheme,
-                 CFLScale=case.CFL, DFLScale=case.DFL)
+                 CFLScale=case.CFL, DFLScale=case.DFL, ref_state_prim=ref_prim)
     try:
         return _time_loop(case, s, mesh.nElems, out_dir, restart, log)
     finally:

@@ -40,7 +40,8 @@ class DGSolver:
 
     def __init__(self, mesh: Mesh, N: int, node_type="gauss-lobatto", split_dg="pi", riemann="roe",
                  kappa=1.4, R=71.42857, Pr=0.72, mu0=6.25e-4, precision="fp64", metrics: Optional[dict] = None,
-                 scheme: LSERK = CARPENTER_RK4_5, CFLScale=0.9, DFLScale=0.9):
+                 scheme: LSERK = CARPENTER_RK4_5, CFLScale=0.9, DFLScale=0.9,
+                 ref_state_prim: Optional[np.ndarray] = None):
         L = _lib.lib()
         self.mesh = mesh
         self.N = N
@@ -66,6 +67,17 @@ class DGSolver:
                              *[_ptr(keep[k]) for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ",
                                                        "NormVec", "TangVec1", "TangVec2", "SurfElem")])
             _lib.check(L.fxg_create(C.byref(self.params), C.byref(m), C.byref(self.ctx)))
+        if getattr(mesh, "nBCSides", 0):
+            # InitBC: BC(nBCSides), BoundaryType(nBCs,3), RefStatePrim(6,nRefState)
+            self._bc = (_f(mesh.BC, np.int32), _f(mesh.BoundaryType, np.int32),
+                        _f(ref_state_prim if ref_state_prim is not None else np.zeros((6, 1))))
+            bc, bt, rp = self._bc
+            rc = L.fxg_set_boundaries(self.ctx, bt.shape[0], _ptr(bc), _ptr(bt), rp.shape[1], _ptr(rp))
+            if rc != 0:
+                msg = L.fxg_last_error().decode()
+                L.fxg_destroy(self.ctx)
+                self.ctx = C.c_void_p()
+                raise _lib.FxgError(msg)
         n = N + 1
         self.shape = (5, n, n, n, mesh.nElems)
         self.dt = 0.0

@@ -56,9 +56,11 @@ typedef struct {
  *   SideToElem(5,nSides)  ElemToSide(2,6,nElems)              (1-based element and side ids)
  *   Metrics_fTilde/gTilde/hTilde(3,0:N,0:N,0:N,nElems)  sJ(0:N,0:N,0:N,nElems)
  *   NormVec/TangVec1/TangVec2(3,0:N,0:N,nSides)  SurfElem(0:N,0:N,nSides)
- * Only conforming inner sides are supported (firstInnerSide = 1, lastInnerSide = nSides;
- * a periodic mesh), i.e. nBCSides = 0, no mortars, a single rank. The library copies the
- * arrays; the caller keeps ownership. */
+ * Sides are boundary sides 1..nBCSides (SideToElem(2,:) = -1, as setLocalSideIDs numbers them,
+ * src/mesh/prepare_mesh.f90:196-222) followed by conforming inner sides: firstInnerSide =
+ * nBCSides+1, lastInnerSide = nSides; no mortars, a single rank. A mesh with boundary sides needs
+ * fxg_set_boundaries before its first right-hand side. The library copies the arrays; the caller
+ * keeps ownership. */
 typedef struct {
   int nElems;
   int nSides;
@@ -90,11 +92,20 @@ const char* fxg_version(void);
 int fxg_create(const fxg_params* params, const fxg_mesh* mesh, fxg_ctx** ctx);
 /* Set-up from geometry: NodeCoords(3,0:NGeo,0:NGeo,0:NGeo,nElems) on equidistant nodes; the
  * metrics of CalcMetrics/CalcSurfMetrics (src/mesh/metrics.f90:161-787, curl form) are computed
- * on the GPU. SideToElem/ElemToSide as in fxg_mesh. */
+ * on the GPU. SideToElem/ElemToSide as in fxg_mesh, boundary sides first. */
 int fxg_create_from_nodes(const fxg_params* params, int nElems, int nSides,
                           const int* SideToElem, const int* ElemToSide, int NGeo,
                           const double* NodeCoords, fxg_ctx** ctx);
 int fxg_destroy(fxg_ctx* ctx);
+/* Boundary conditions of the boundary sides (InitBC, src/equations/navierstokes/idealgas/
+ * getboundaryflux.f90:64-218, with FLEXI's arrays): BC(nBCSides) the boundary index of each side
+ * (mesh_vars.f90), BoundaryType(nBCs,3) = (type, state, alpha) in Fortran layout, RefStatePrim(6,
+ * nRefState) = (rho,u,v,w,p,T) per RefState (equation.f90:126-141). Types: 2 Dirichlet with
+ * state = RefState index (>= 1), 3 adiabatic wall, 4 isothermal wall (temperature of RefState
+ * `state`), 9 slip wall (getboundaryflux.f90:259-465, 480-789, 855-939). Meshes with boundary
+ * sides run the four-kernel path. No-op for a mesh without boundary sides. */
+int fxg_set_boundaries(fxg_ctx* ctx, int nBCs, const int* BC, const int* BoundaryType, int nRefState,
+                       const double* RefStatePrim);
 
 /* Host <-> device state transfer in FLEXI layout U(5,0:N,0:N,0:N,nElems). */
 int fxg_set_state(fxg_ctx* ctx, const double* U);

@@ -36,6 +36,21 @@ int fxg_io_mesh_size(const char* path, int* nElems, int* nSides, int* NGeo);
 int fxg_io_read_mesh(const char* path, int nElems, int nSides, int NGeo, int* SideToElem, int* ElemToSide,
                      double* NodeCoords);
 
+/* The same for meshes with boundary sides (ReadBCs + ReadMesh + setLocalSideIDs, mesh_readin.f90:
+ * 70-184, 397-470, prepare_mesh.f90:196-243). userNames[nUser] / userTypes[2*nUser] are the
+ * BoundaryName / BoundaryType (type, state) pairs of FLEXI's parameter file, which replace the type
+ * and state of the file's boundary of the same name (case-insensitive); nUser may be 0. Sides of
+ * boundaries whose type is not 0, 1 or 100 are boundary sides: numbered first (1..nBCSides), flip 0,
+ * no neighbour (SideToElem(2,:) = -1). nBCs is the number of boundaries in the file. */
+int fxg_io_mesh_size_bc(const char* path, int nUser, const char* const* userNames, const int* userTypes, int* nElems,
+                        int* nSides, int* NGeo, int* nBCSides, int* nBCs);
+/* Arrays as fxg_io_read_mesh plus BC(nBCSides) (the boundary index of each boundary side, fillMeshInfo
+ * prepare_mesh.f90:717) and BoundaryType(nBCs,3) = (type, state, alpha) in FLEXI's Fortran layout
+ * (mesh_readin.f90:168-173); BCNames may be NULL, else nBCs strings of nameLen bytes each. */
+int fxg_io_read_mesh_bc(const char* path, int nUser, const char* const* userNames, const int* userTypes, int nElems,
+                        int nSides, int NGeo, int nBCSides, int nBCs, int* SideToElem, int* ElemToSide,
+                        double* NodeCoords, int* BC, int* BoundaryType, char* BCNames, int nameLen);
+
 /* Sizes of a State file's DG_Solution (GetDataProps, hdf5_input.f90:307-360). */
 int fxg_io_state_size(const char* path, int* nVar, int* N, int* nElems);
 

@@ -282,11 +282,118 @@ struct or_ctx {
   std::vector<double> UPrim, U_master, U_slave, UPrim_master, UPrim_slave, Flux_master, Flux_slave;
   std::vector<double> gx, gy, gz, gx_m, gy_m, gz_m, gx_s, gy_s, gz_s, Ut_tmp_rk;
   std::vector<double> MetricsAdv, MetricsVisc;
+  // boundary sides 1..nBCSides (mesh_vars.f90: BC, BoundaryType; equation_vars.f90: RefStatePrim)
+  int nBCSides = 0, nBCs = 0, nRefState = 0;
+  std::vector<int> BC, BoundaryType;  // BC(nBCSides), BoundaryType(nBCs,3): type, state, alpha
+  std::vector<double> RefStatePrim;   // (6,nRefState)
 };
 
 namespace {
 
 inline int vidx(const or_ctx* c, int i, int j, int k) { return i + c->n * (j + c->n * k); }
+
+// PRESSURE_RIEMANN, idealgas/eos.f90:561-584: pressure of the local Riemann problem against a
+// wall, P in the side frame (VEL1 = normal velocity).
+double pressure_riemann(const EosC& e, const double* P) {
+  const double kappaFac = 2. * e.kappa * e.sKappaM1;
+  if (P[VEL1] <= 0.) {  // rarefaction
+    return P[PRES] * std::pow(std::fmax(0.0001, 1. + 0.5 * e.kappaM1 * P[VEL1] / std::sqrt(e.kappa * P[PRES] / P[DENS])),
+                              kappaFac);
+  }
+  const double sKappaP1 = 1. / (e.kappa + 1.);  // shock
+  const double ar = 2. * sKappaP1 / P[DENS];
+  const double br = e.kappaM1 * sKappaP1 * P[PRES];
+  return P[PRES] + P[VEL1] / ar * 0.5 * (P[VEL1] + std::sqrt(P[VEL1] * P[VEL1] + 4. * ar * (P[PRES] + br)));
+}
+
+// GetBoundaryState, idealgas/getboundaryflux.f90:224-472, for BC types 2 (RefState, BCState >= 1),
+// 3 (adiabatic wall), 4 (isothermal wall) and 9 (slip wall); ref = RefStatePrim(:,BCState).
+void boundary_state(const EosC& e, int type, const double* ref, const double* Pm, const double* nv,
+                    const double* t1, const double* t2, double* Pb) {
+  if (type == 2) {
+    for (int v = 0; v < 6; ++v) Pb[v] = ref[v];
+    return;
+  }
+  Pb[DENS] = Pm[DENS];
+  Pb[VEL1] = Pm[VEL1] * nv[0] + Pm[VEL2] * nv[1] + Pm[VEL3] * nv[2];
+  Pb[VEL2] = Pm[VEL1] * t1[0] + Pm[VEL2] * t1[1] + Pm[VEL3] * t1[2];
+  Pb[VEL3] = Pm[VEL1] * t2[0] + Pm[VEL2] * t2[1] + Pm[VEL3] * t2[2];
+  Pb[PRES] = Pm[PRES];
+  Pb[TEMP] = Pm[TEMP];
+  if (type == 3 || type == 4) {
+    Pb[PRES] = pressure_riemann(e, Pb);
+    Pb[VEL1] = Pb[VEL2] = Pb[VEL3] = 0.;
+    Pb[TEMP] = type == 3 ? Pm[TEMP] : ref[TEMP];
+    Pb[DENS] = Pb[PRES] / (Pb[TEMP] * e.R);
+  } else {  // 9
+    Pb[PRES] = pressure_riemann(e, Pb);
+    Pb[VEL1] = 0.;
+    Pb[DENS] = Pm[DENS];
+    Pb[TEMP] = Pb[PRES] / (Pb[DENS] * e.R);
+  }
+  const double a = Pb[VEL1], b = Pb[VEL2], c = Pb[VEL3];
+  for (int d = 0; d < 3; ++d) Pb[VEL1 + d] = a * nv[d] + b * t1[d] + c * t2[d];
+}
+
+// Lifting_GetBoundaryFlux, idealgas/getboundaryflux.f90:855-939 (strong form, lifting variables
+// u,v,w,T), before the SurfElem factor: the boundary value minus the inner one.
+void lifting_boundary_flux(int type, const double* Pm, const double* Pb, double* F) {
+  const int PL[4] = {VEL1, VEL2, VEL3, TEMP};
+  for (int v = 0; v < 4; ++v) {
+    double f;
+    if (type == 2) f = 0.5 * (Pm[PL[v]] + Pb[PL[v]]);
+    else if (type == 3 || type == 4) f = v < 3 ? 0. : Pb[TEMP];
+    else f = v < 3 ? Pb[PL[v]] : Pm[TEMP];
+    F[v] = f - Pm[PL[v]];
+  }
+}
+
+// GetBoundaryFlux, idealgas/getboundaryflux.f90:480-789, before the SurfElem factor.
+void boundary_flux(const or_ctx* c, int type, const double* Pm, const double* Pb, const double* gx,
+                   const double* gy, const double* gz, const double* nv, const double* t1, const double* t2,
+                   double* F) {
+  const EosC& e = c->e;
+  if (type == 2) {  // Riemann-type: Riemann solver against the boundary state + ViscousFlux
+    double Um[5], Ub[5];
+    prim_to_cons(e, Pm, Um);
+    prim_to_cons(e, Pb, Ub);
+    riemann_point(e, c->p.riemann, c->p.splitDG, Um, Ub, Pm, Pb, nv, t1, t2, F);
+    double fL[5], gL[5], hL[5], fR[5], gR[5], hR[5];
+    eval_diff_flux(e, Pm, gx, gy, gz, fL, gL, hL);
+    eval_diff_flux(e, Pb, gx, gy, gz, fR, gR, hR);
+    for (int v = 0; v < 5; ++v)
+      F[v] = F[v] + 0.5 * (nv[0] * (fL[v] + fR[v]) + nv[1] * (gL[v] + gR[v]) + nv[2] * (hL[v] + hR[v]));
+    return;
+  }
+  // walls: Euler flux with zero normal velocity, F = (0, p n, 0)
+  F[DENS] = 0.;
+  for (int d = 0; d < 3; ++d) F[MOM1 + d] = Pb[PRES] * nv[d];
+  F[ENER] = 0.;
+  double fd[5], gd[5], hd[5];
+  if (type == 3 || type == 4) {
+    eval_diff_flux(e, Pb, gx, gy, gz, fd, gd, hd);
+    if (type == 3) fd[ENER] = gd[ENER] = hd[ENER] = 0.;
+  } else {  // 9: gradients without their normal component, (I - n n^T) grad
+    double M[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) M[a][b] = a == b ? 1. - nv[a] * nv[a] : -nv[a] * nv[b];
+    double qx[4], qy[4], qz[4];
+    for (int v = 0; v < 4; ++v) {
+      qx[v] = M[0][0] * gx[v] + M[0][1] * gy[v] + M[0][2] * gz[v];
+      qy[v] = M[1][0] * gx[v] + M[1][1] * gy[v] + M[1][2] * gz[v];
+      qz[v] = M[2][0] * gx[v] + M[2][1] * gy[v] + M[2][2] * gz[v];
+    }
+    eval_diff_flux(e, Pb, qx, qy, qz, fd, gd, hd);
+  }
+  for (int v = 1; v < 5; ++v) F[v] = F[v] + nv[0] * fd[v] + nv[1] * gd[v] + nv[2] * hd[v];
+}
+
+const double* bc_ref(const or_ctx* c, int s) {
+  const int iBC = c->BC[s] - 1;
+  const int state = c->BoundaryType[iBC + c->nBCs];
+  return state >= 1 ? &c->RefStatePrim[6 * (state - 1)] : nullptr;
+}
+inline int bc_type(const or_ctx* c, int s) { return c->BoundaryType[c->BC[s] - 1]; }
 
 // ProlongToFace, interpolation/prolongtoface.t90:21-144 (EvalElemFaceG / EvalElemFaceGL).
 void prolong_to_face(const or_ctx* c, int nVar, const double* Uvol, double* Um, double* Us) {
@@ -430,8 +537,25 @@ void lifting_br1(or_ctx* c) {
         }
   }
   // Lifting_FillFlux (dir=0) and Lifting_FillFlux_NormVec, lifting_fillflux.t90:40-117,189-239
-  std::vector<double> flux(4 * nf);
-  for (int s = 0; s < c->nSides; ++s) {
+  // Lifting_FillFlux_BC (lifting_fillflux.t90:133-180) for sides 1..nBCSides
+  for (int s = 0; s < c->nBCSides; ++s) {
+    const int type = bc_type(c, s);
+    const double* ref = bc_ref(c, s);
+    for (int pq = 0; pq < nf; ++pq) {
+      const size_t fi = (size_t)nf * s + pq;
+      double Pb[6], F[4];
+      boundary_state(c->e, type, ref, &c->UPrim_master[6 * fi], &c->NormVec[3 * fi], &c->TangVec1[3 * fi],
+                     &c->TangVec2[3 * fi], Pb);
+      lifting_boundary_flux(type, &c->UPrim_master[6 * fi], Pb, F);
+      for (int v = 0; v < 4; ++v) {
+        const double f = F[v] * c->SurfElem[fi];
+        c->gx_m[v + 4 * fi] = f * c->NormVec[0 + 3 * fi];
+        c->gy_m[v + 4 * fi] = f * c->NormVec[1 + 3 * fi];
+        c->gz_m[v + 4 * fi] = f * c->NormVec[2 + 3 * fi];
+      }
+    }
+  }
+  for (int s = c->nBCSides; s < c->nSides; ++s) {
     for (int pq = 0; pq < nf; ++pq) {
       const size_t fi = (size_t)nf * s + pq;
       for (int v = 0; v < 4; ++v) {
@@ -443,7 +567,6 @@ void lifting_br1(or_ctx* c) {
       }
     }
   }
-  (void)flux;
   // Lifting_SurfInt_BR1 with the same flux on master and slave (strong form)
   surf_int(c, 4, c->gx_m.data(), c->gx_m.data(), +1.0, c->gx.data());
   surf_int(c, 4, c->gy_m.data(), c->gy_m.data(), +1.0, c->gy.data());
@@ -574,8 +697,22 @@ void vol_int(or_ctx* c, const double* Uall, double* Utall) {
 void fill_flux(or_ctx* c) {
   const int nf = c->nf;
   const int split = c->p.splitDG;
+  // 2. boundary sides (fillflux.f90:172-190), then 3. times SurfElem
+  for (int s = 0; s < c->nBCSides; ++s) {
+    const int type = bc_type(c, s);
+    const double* ref = bc_ref(c, s);
+    for (int pq = 0; pq < nf; ++pq) {
+      const size_t fi = (size_t)nf * s + pq;
+      const double *Pm = &c->UPrim_master[6 * fi], *nv = &c->NormVec[3 * fi], *t1 = &c->TangVec1[3 * fi],
+                   *t2 = &c->TangVec2[3 * fi];
+      double Pb[6], F[5];
+      boundary_state(c->e, type, ref, Pm, nv, t1, t2, Pb);
+      boundary_flux(c, type, Pm, Pb, &c->gx_m[4 * fi], &c->gy_m[4 * fi], &c->gz_m[4 * fi], nv, t1, t2, F);
+      for (int v = 0; v < 5; ++v) c->Flux_master[v + 5 * fi] = F[v] * c->SurfElem[fi];
+    }
+  }
 #pragma omp parallel for schedule(static)
-  for (int s = 0; s < c->nSides; ++s) {
+  for (int s = c->nBCSides; s < c->nSides; ++s) {
     for (int pq = 0; pq < nf; ++pq) {
       const size_t fi = (size_t)nf * s + pq;
       double F[5];
@@ -679,6 +816,25 @@ or_ctx* or_init(const or_params* p, int nElems, int nSides, const int* SideToEle
 
 void or_free(or_ctx* c) { delete c; }
 
+int or_set_boundaries(or_ctx* c, int nBCSides, const int* BC, int nBCs, const int* BoundaryType, int nRefState,
+                      const double* RefStatePrim) {
+  if (nBCSides < 0 || nBCSides > c->nSides) return 1;
+  for (int s = 0; s < nBCSides; ++s) {
+    if (c->S2E[1 + 5 * s] > 0) return 2;  // a boundary side has no neighbour element
+    if (BC[s] < 1 || BC[s] > nBCs) return 3;
+    const int type = BoundaryType[BC[s] - 1], state = BoundaryType[BC[s] - 1 + nBCs];
+    if (type != 2 && type != 3 && type != 4 && type != 9) return 4;
+    if ((type == 2 || type == 4) && (state < 1 || state > nRefState)) return 5;
+  }
+  c->nBCSides = nBCSides;
+  c->nBCs = nBCs;
+  c->nRefState = nRefState;
+  c->BC.assign(BC, BC + nBCSides);
+  c->BoundaryType.assign(BoundaryType, BoundaryType + 3 * nBCs);
+  c->RefStatePrim.assign(RefStatePrim, RefStatePrim + 6 * nRefState);
+  return 0;
+}
+
 void or_dg_time_derivative(or_ctx* c, double t, const double* U, double* Ut) {
   (void)t;
   const size_t np = c->np, K = c->nElems, nf = c->nf;
@@ -691,7 +847,7 @@ void or_dg_time_derivative(or_ctx* c, double t, const double* U, double* Ut) {
 #pragma omp parallel for schedule(static)
   for (long f = 0; f < (long)(nf * c->nSides); ++f) {
     cons_to_prim(c->e, &c->U_master[5 * f], &c->UPrim_master[6 * f]);
-    cons_to_prim(c->e, &c->U_slave[5 * f], &c->UPrim_slave[6 * f]);
+    if (f >= (long)nf * c->nBCSides) cons_to_prim(c->e, &c->U_slave[5 * f], &c->UPrim_slave[6 * f]);
   }
   // 6. Lifting
   lifting_br1(c);

@@ -66,12 +66,20 @@ void or_eval_diff_flux_point(const or_params* p, const double* UPrim, const doub
 typedef struct or_ctx or_ctx;
 /* Mesh arrays in FLEXI layout. SideToElem(5,nSides), ElemToSide(2,6,nElems) hold 1-based ids.
  * Metrics_*(3,0:N,0:N,0:N,nElems), sJ(0:N,0:N,0:N,nElems), NormVec/TangVec*(3,0:N,0:N,nSides),
- * SurfElem(0:N,0:N,nSides). All sides are inner sides (periodic meshes; nBCSides = 0). */
+ * SurfElem(0:N,0:N,nSides). Sides 1..nBCSides of or_set_boundaries are boundary sides (no
+ * neighbour, SideToElem(2,:) = -1); all others are inner sides. */
 or_ctx* or_init(const or_params* p, int nElems, int nSides, const int* SideToElem,
                 const int* ElemToSide, const double* Mf, const double* Mg, const double* Mh,
                 const double* sJ, const double* NormVec, const double* TangVec1,
                 const double* TangVec2, const double* SurfElem);
 void or_free(or_ctx* c);
+/* Boundary sides 1..nBCSides (numbered first, as setLocalSideIDs does, prepare_mesh.f90:196-222):
+ * BC(nBCSides) the boundary index of each, BoundaryType(nBCs,3) = (type, state, alpha) per
+ * boundary (mesh_readin.f90:167-173), RefStatePrim(6,nRefState) (equation.f90:126-141). Types 2
+ * (BCState >= 1), 3, 4 and 9 (getboundaryflux.f90). Returns 0, or a nonzero code for an invalid
+ * side, index, type or state. */
+int or_set_boundaries(or_ctx* c, int nBCSides, const int* BC, int nBCs, const int* BoundaryType, int nRefState,
+                      const double* RefStatePrim);
 void or_set_threads(int n);
 /* DGTimeDerivative_weakForm (dg.f90:221-678): U(5,0:N,0:N,0:N,nElems) -> Ut. */
 void or_dg_time_derivative(or_ctx* c, double t, const double* U, double* Ut);

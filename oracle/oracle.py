@@ -62,6 +62,8 @@ def _setup(L):
     L.or_init.restype = v
     L.or_init.argtypes = [P, C.c_int, C.c_int, _ip, _ip] + [_dp] * 8
     L.or_free.argtypes = [v]
+    L.or_set_boundaries.restype = C.c_int
+    L.or_set_boundaries.argtypes = [v, C.c_int, _ip, C.c_int, _ip, C.c_int, _dp]
     L.or_set_threads.argtypes = [C.c_int]
     L.or_dg_time_derivative.argtypes = [v, C.c_double, _dp, _dp]
     L.or_get_gradients.argtypes = [v, _dp, _dp, _dp]
@@ -149,7 +151,7 @@ def rk_coefs():
 class Solver:
     """The oracle's DG operator on one mesh (FLEXI layout)."""
 
-    def __init__(self, prm: Params, mesh, metrics):
+    def __init__(self, prm: Params, mesh, metrics, ref_state_prim=None):
         self.p = prm
         self.N = prm.N
         self.mesh = mesh
@@ -161,6 +163,13 @@ class Solver:
                                  f(mesh.ElemToSide), *self._keep)
         n = self.N + 1
         self.shape = (5, n, n, n, mesh.nElems)
+        if getattr(mesh, "nBCSides", 0):
+            ref = np.asfortranarray(ref_state_prim if ref_state_prim is not None else np.zeros((6, 1)))
+            bt = np.asfortranarray(mesh.BoundaryType, dtype=np.int32)
+            rc = lib().or_set_boundaries(self.ctx, mesh.nBCSides, np.asfortranarray(mesh.BC, dtype=np.int32),
+                                         bt.shape[0], bt, ref.shape[1], ref)
+            if rc != 0:
+                raise ValueError(f"or_set_boundaries: invalid boundary data (code {rc})")
 
     def __del__(self):
         try:

@@ -1,0 +1,257 @@
+"""Boundary conditions (FLEXI's GetBoundaryFlux / Lifting_GetBoundaryFlux) and meshes with
+boundary sides.
+
+tests/golden/cavity/ holds unmodified copies of the reference's
+regressioncheck/checks/parabolic/cavity_3D files: the HOPR mesh cavity4x4x4_mesh.h5 (4^3 elements
+on the unit cube, six named boundaries), parameter.ini (N=2, Gauss nodes, weak form,
+RoeEntropyFix, BR1, mu0=0.01, isothermal walls and a Dirichlet lid moving with u=1, t=0..1), and
+reggie_cavity_Re100_State_0000001.000000000.h5, the State FLEXI wrote at t=1, which its regression
+check compares with h5diff at 1e-12 absolute.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from flexi_amd import equation
+from flexi_amd import run as frun
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CAV = os.path.join(ROOT, "tests", "golden", "cavity")
+HD = os.path.join(ROOT, "tests", "golden", "hdf5")
+CAVITY_BCS = [("BC_wall_left", (4, 1)), ("BC_wall_right", (4, 1)), ("BC_free", (2, 1))]
+CAVITY_REF = [[1.0, 1.0, 0.0, 0.0, 71.4285714286], [1.0, 0.0, 0.0, 0.0, 71.4285714286]]
+
+
+def fio():
+    from flexi_amd import io
+    return io
+
+
+def cavity_mesh(boundaries=CAVITY_BCS):
+    return fio().read_mesh_bc(os.path.join(CAV, "cavity4x4x4_mesh.h5"), boundaries)
+
+
+def test_cavity_mesh_boundary_sides():
+    m = cavity_mesh()
+    assert (m.nElems, m.nSides, m.nBCSides) == (64, 240, 96)
+    assert m.BCNames == ["BC_zminus", "BC_zplus", "BC_wall_lower", "BC_free", "BC_wall_left", "BC_wall_right"]
+    # the file's types, with the parameter file's BoundaryType for three of them (BC_free was (2,0))
+    assert m.BoundaryType[:, 0].tolist() == [4, 4, 4, 2, 4, 4]
+    assert m.BoundaryType[:, 1].tolist() == [1, 1, 1, 1, 1, 1]
+    assert np.bincount(m.BC, minlength=7)[1:].tolist() == [16] * 6
+    s2e = m.SideToElem
+    # boundary sides first (setLocalSideIDs): no neighbour; inner sides have both
+    assert np.all(s2e[1, :96] == -1) and np.all(s2e[3, :96] == -1) and np.all(s2e[0, :96] >= 1)
+    assert np.all(s2e[1, 96:] >= 1) and np.all(s2e[4, 96:] >= 1)
+    # every element's side in ElemToSide points back through SideToElem
+    for e in range(m.nElems):
+        for ls in range(6):
+            sid, flip = m.ElemToSide[:, ls, e]
+            col = s2e[:, sid - 1]
+            if flip == 0:
+                assert col[0] == e + 1 and col[2] == ls + 1
+            else:
+                assert col[1] == e + 1 and col[3] == ls + 1 and col[4] == flip
+    # each boundary side lies on the face of the unit cube its name says (corner nodes of the side)
+    from flexi_amd.mesh import side_to_vol
+    plane = {"BC_zminus": (2, 0.0), "BC_zplus": (2, 1.0), "BC_wall_lower": (1, 0.0), "BC_free": (1, 1.0),
+             "BC_wall_left": (0, 0.0), "BC_wall_right": (0, 1.0)}
+    for s in range(96):
+        d, val = plane[m.BCNames[m.BC[s] - 1]]
+        e, ls = s2e[0, s] - 1, s2e[2, s]
+        for q in range(2):
+            for p in range(2):
+                x = m.NodeCoords[(slice(None),) + side_to_vol(1, 0, p, q, 0, ls) + (e,)]
+                assert abs(x[d] - val) < 1e-12, (s, m.BCNames[m.BC[s] - 1], x)
+
+
+def test_boundary_mesh_errors_and_periodic_agreement():
+    io = fio()
+    with pytest.raises(RuntimeError, match="not found"):
+        cavity_mesh([("BC_nowhere", (4, 1))])
+    with pytest.raises(RuntimeError, match="periodic"):
+        cavity_mesh([("BC_free", (1, 0))])
+    with pytest.raises(RuntimeError, match="boundary sides are not supported"):
+        io.read_mesh(os.path.join(CAV, "cavity4x4x4_mesh.h5"))
+    # a periodic mesh reads the same through both functions
+    p = os.path.join(HD, "CART_HEX_PERIODIC_004_mesh.h5")
+    a, b = io.read_mesh(p), io.read_mesh_bc(p)
+    assert b.nBCSides == 0 and np.array_equal(a.SideToElem, b.SideToElem)
+    assert np.array_equal(a.ElemToSide, b.ElemToSide) and np.array_equal(a.NodeCoords, b.NodeCoords)
+    # the freestream fixture's Dirichlet box: all 6*4 outer faces are boundary sides
+    c = io.read_mesh_bc(os.path.join(HD, "cartbox3D_mesh.h5"))
+    assert c.nElems == 8 and c.nBCSides == 24 and c.nSides == 24 + 12
+
+
+def oracle_solver(mesh, N, node, split, riemann, mu0=0.01, R=1.0, ref=CAVITY_REF):
+    prm = orc.params(N, node, split, riemann, kappa=1.4, R=R, Pr=0.72, mu0=mu0)
+    met = orc.calc_metrics(N, node, mesh)
+    return orc.Solver(prm, mesh, met, equation.ref_state_prim(ref, R)), met
+
+
+def run_to(S, U, t_end, CFL, DFL, N, node):
+    cfl, dfl = orc.fill_cfl_dfl(N, node, CFL, DFL)
+    t, R = 0.0, orc.F(S.shape)
+    it = 0
+    while True:
+        dt_min, err, _ = S.calc_timestep(U, cfl, dfl)
+        assert err == 0
+        dt, _, fin = frun.update_time_step(dt_min, t, t_end, t_end)
+        S.step(U, t, dt, R)
+        t += dt
+        it += 1
+        if fin:
+            return t, it
+
+
+def test_oracle_reproduces_flexis_cavity_state():
+    """The oracle's lid-driven cavity (isothermal walls, Dirichlet lid) from rest to t=1: FLEXI's
+    State at t=1 to its regression tolerance, 1e-12 absolute (observed 1.7e-13 on the energy,
+    |E| ~ 180). This pins GetBoundaryState/GetBoundaryFlux/Lifting_GetBoundaryFlux for types 2
+    and 4, the weak-form Gauss operator at N=2 and the time loop's dt rules over 353 steps."""
+    m = cavity_mesh()
+    S, _ = oracle_solver(m, 2, orc.NODE_G, 0, orc.RIEMANN_ROEENTROPYFIX)
+    U0 = equation.prim_to_cons(equation.ref_state_prim(CAVITY_REF, 1.0)[:, 1], 1.4)
+    U = np.asfortranarray(np.broadcast_to(U0.reshape(5, 1, 1, 1, 1), S.shape)).copy(order="F")
+    t, it = run_to(S, U, 1.0, 0.99, 0.4, 2, orc.NODE_G)
+    st = fio().read_state(os.path.join(CAV, "reggie_cavity_Re100_State_0000001.000000000.h5"))
+    assert st["time"] == 1.0 and st["N"] == 2 and st["node_type"] == "GAUSS"
+    err = np.max(np.abs(st["U"] - U))
+    print(f"{it} steps, max |U - U_FLEXI| = {err:.2e}")
+    assert t == 1.0 and err <= 1e-12
+
+
+def smooth_state(xgp, amp=0.05, seed=0):
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(1.0, 3.0, size=(5, 3))
+    ph = rng.uniform(0, 2 * np.pi, size=5)
+    x, y, z = xgp
+    w = [np.sin(k[v, 0] * x + k[v, 1] * y + k[v, 2] * z + ph[v]) for v in range(5)]
+    prim = np.stack([1.0 + amp * w[0], 0.3 * w[1], 0.2 * w[2], 0.1 * w[3], 71.43 * (1.0 + amp * w[4])])
+    return np.asfortranarray(equation.prim_to_cons(prim, 1.4))
+
+
+def weighted_sum(Ut, sJ, w):
+    W = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    return np.einsum("vijke,ijk,ijke->v", Ut, W, 1.0 / sJ)
+
+
+@pytest.mark.parametrize("wall", [3, 4, 9])
+def test_oracle_walls_conserve_mass(wall):
+    """A box closed by walls of one type: no mass crosses a wall (Flux(DENS)=0), so the
+    quadrature of the density's time derivative vanishes to round-off for any state; at
+    adiabatic walls (3) the energy flux is zero too."""
+    bcs = [(n, (wall, 1)) for n in ("BC_zminus", "BC_zplus", "BC_wall_lower", "BC_free", "BC_wall_left",
+                                    "BC_wall_right")]
+    m = cavity_mesh(bcs)
+    S, met = oracle_solver(m, 3, orc.NODE_G, 0, orc.RIEMANN_ROE)
+    U = smooth_state(met["Elem_xGP"])
+    Ut = S.rhs(U)
+    w = orc.dg_basis(3, orc.NODE_G)["wGP"]
+    tot = weighted_sum(Ut, met["sJ"], w)
+    scale = weighted_sum(np.abs(Ut), met["sJ"], w)
+    print(wall, tot / scale)
+    assert abs(tot[0]) <= 1e-13 * scale[0]
+    if wall == 3:
+        assert abs(tot[4]) <= 1e-13 * scale[4]
+    else:
+        assert abs(tot[4]) > 1e-10 * scale[4]  # heat or work crosses these walls
+
+
+@pytest.mark.parametrize("bc", [(2, 2), (3, 1), (4, 2), (9, 1)])
+def test_oracle_rest_state_is_steady(bc):
+    """Fluid at rest at the RefState's temperature and pressure, in a box with one boundary type
+    all round: no flux anywhere but the pressure (a wall's Riemann pressure at zero normal velocity
+    is the inner pressure), so the state is steady to round-off."""
+    bcs = [(n, bc) for n in ("BC_zminus", "BC_zplus", "BC_wall_lower", "BC_free", "BC_wall_left", "BC_wall_right")]
+    m = cavity_mesh(bcs)
+    S, _ = oracle_solver(m, 2, orc.NODE_G, 0, orc.RIEMANN_ROEENTROPYFIX)
+    U0 = equation.prim_to_cons(equation.ref_state_prim(CAVITY_REF, 1.0)[:, 1], 1.4)
+    U = np.asfortranarray(np.broadcast_to(U0.reshape(5, 1, 1, 1, 1), S.shape)).copy(order="F")
+    Ut = S.rhs(U)
+    assert np.max(np.abs(Ut)) < 1e-10 * 178.0
+
+
+def test_cavity_parameter_file():
+    prm = frun.parse_parameter_file(os.path.join(CAV, "parameter.ini"))
+    c = frun.Case(prm, CAV)
+    assert c.boundaries == CAVITY_BCS
+    assert (c.N, c.riemann, c.split, c.mu0, c.R, c.tend, c.analyze_dt, c.CFL, c.DFL) == \
+        (2, "roeentropyfix", None, 0.01, 1.0, 1.0, 1.0, 0.99, 0.4)
+    assert c.ini_exact == 1 and c.ini_ref == 2 and c.ref_states == CAVITY_REF
+
+
+# ------------------------------------------------------------------------------------------------
+# GPU
+
+
+def gpu_solver(mesh, N, node, split, riemann, precision="fp64", mu0=0.01):
+    from flexi_amd.solver import DGSolver
+    return DGSolver(mesh, N, node_type=node, split_dg=split, riemann=riemann, kappa=1.4, R=1.0, Pr=0.72, mu0=mu0,
+                    precision=precision, ref_state_prim=equation.ref_state_prim(CAVITY_REF, 1.0), CFLScale=0.99,
+                    DFLScale=0.4)
+
+
+CASES = [(2, "gauss", None, "roeentropyfix"), (3, "gauss", None, "lf"), (4, "gauss-lobatto", None, "roe"),
+         (5, "gauss-lobatto", "pi", "roe")]
+BC_SETS = {
+    "cavity": CAVITY_BCS,
+    "adiabatic+slip": [("BC_zminus", (9, 1)), ("BC_zplus", (9, 1)), ("BC_wall_lower", (3, 1)), ("BC_free", (2, 1)),
+                       ("BC_wall_left", (3, 1)), ("BC_wall_right", (4, 2))],
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bcs", list(BC_SETS))
+@pytest.mark.parametrize("N,node,split,riemann", CASES)
+def test_gpu_rhs_with_boundaries_matches_oracle(N, node, split, riemann, bcs):
+    """Right-hand side and lifted gradients of the four-kernel GPU path on the cavity mesh with
+    every supported boundary type, against the oracle, for a smooth non-trivial state."""
+    m = cavity_mesh(BC_SETS[bcs])
+    onode = orc.NODE_G if node == "gauss" else orc.NODE_GL
+    oriem = {"lf": orc.RIEMANN_LF, "roe": orc.RIEMANN_ROE, "roeentropyfix": orc.RIEMANN_ROEENTROPYFIX}[riemann]
+    S, met = oracle_solver(m, N, onode, 1 if split else 0, oriem)
+    U = smooth_state(met["Elem_xGP"], seed=N)
+    ref = S.rhs(U)
+    gref = S.gradients()
+    g = gpu_solver(m, N, node, split, riemann)
+    try:
+        assert g.kernel_pipeline() == "classic"
+        out = g.rhs(U)
+        gg = g.gradients()
+    finally:
+        g.close()
+    err = np.max(np.abs(out - ref)) / np.max(np.abs(ref))
+    gerr = max(np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(gg, gref))
+    print(f"N={N} {node} {split} {riemann} {bcs}: rhs {err:.2e} grads {gerr:.2e}")
+    assert err <= 1e-12 and gerr <= 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_cavity_run_matches_flexis_state(tmp_path):
+    """The reference's cavity_3D case through the parameter-file driver on the GPU (353 steps to
+    t=1): FLEXI's State at t=1 to its h5diff tolerance, 1e-12 absolute; and the FP32-bulk path
+    within 1e-6 relative L2 of it."""
+    io = fio()
+    prm = frun.parse_parameter_file(os.path.join(CAV, "parameter.ini"))
+    ref = io.read_state(os.path.join(CAV, "reggie_cavity_Re100_State_0000001.000000000.h5"))
+    out = {}
+    for prec in ("fp64", "fp32-bulk"):
+        d = tmp_path / prec
+        d.mkdir()
+        case = frun.Case(prm, CAV, node_type="gauss")
+        res = frun.run(case, str(d), precision=prec, log=lambda s: None)
+        names = [os.path.basename(p) for p in res["states"]]
+        assert names == ["cavity_Re100_State_0000000.000000000.h5", "cavity_Re100_State_0000001.000000000.h5"]
+        st = io.read_state(res["states"][-1])
+        assert st["time"] == 1.0 and st["node_type"] == "GAUSS"
+        out[prec] = (st["U"], res["iterations"])
+    U64, it = out["fp64"]
+    err = np.max(np.abs(U64 - ref["U"]))
+    U32 = out["fp32-bulk"][0]
+    rel = np.linalg.norm(U32 - ref["U"]) / np.linalg.norm(ref["U"])
+    print(f"{it} steps: FP64 max |U - U_FLEXI| = {err:.2e}; FP32-bulk rel L2 {rel:.2e}")
+    assert it == 353 and err <= 1e-12
+    assert rel <= 1e-6

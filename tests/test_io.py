@@ -31,7 +31,7 @@ def test_io_library_exports_every_header_symbol():
     fio = io()
     text = open(os.path.join(ROOT, "include", "fxg_io.h")).read()
     syms = sorted(set(re.findall(r"\b(fxg_io_[a-z0-9_]+)\s*\(", text)))
-    assert len(syms) == 6
+    assert len(syms) == 8
     out = subprocess.run(["nm", "-D", "--defined-only", fio.IO_LIB_PATH], capture_output=True, text=True).stdout
     for s in syms:
         assert re.search(rf"\bT {s}$", out, re.M), s
