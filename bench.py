@@ -128,9 +128,10 @@ def fp32_accuracy():
 
 def adjoint_ratio():
     """Gradient of the kinetic energy at T w.r.t. U(0): TGV 16^3, N=5 Gauss-Lobatto split PI, Roe,
-    200 steps (dt frozen from the primal run), checkpoints every 20 steps. `ratio` = time of the
-    adjoint (backward) call, which recomputes the stage states of each 20-step segment from the
-    forward's checkpoints and runs the reverse stages, / time of the 200 primal forward steps."""
+    200 steps (dt frozen from the primal run). `ratio` = time of the adjoint (backward) call / time
+    of the 200 primal forward steps, with the trajectory the forward stores in HBM (71 GB: every
+    stage state and its face states); `checkpointed` = the same with checkpoints every 20 steps,
+    the backward recomputing the stage states of each segment."""
     import torch
     from flexi_amd import testcase
     from flexi_amd.adjoint import primal_dts, to_tensor
@@ -161,14 +162,32 @@ def adjoint_ratio():
     t_fwd_ck = timed(lambda: s.forward_n_steps_dev(U.data_ptr(), dts, ckpt.data_ptr(), every))
     ubar = torch.ones_like(U0d)
     t_bwd = timed(lambda: s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts, every, ckpt.data_ptr()))
+    g_ck = ubar.clone()
+    del ckpt
+    # trajectory mode: the forward stores every stage state and its face states (out of place, no
+    # extra bytes moved), the backward runs the reverse stages only
+    traj = torch.empty(s.adjoint_storage(len(dts), 0), dtype=U0d.dtype, device=U0d.device)
+    U = U0d.clone()
+    t_fwd_tr = timed(lambda: s.forward_n_steps_dev(U.data_ptr(), dts, traj.data_ptr(), 0))
+    ubar = torch.ones_like(U0d)
+    t_bwd_tr = timed(lambda: s.adjoint_n_steps_dev(U0d.data_ptr(), ubar.data_ptr(), dts, 0, traj.data_ptr()))
+    same = bool(torch.equal(ubar, g_ck))
+    traj_bytes = traj.numel() * 8
+    del traj
     state = U0.nbytes
     face_states = 10 * (5 + 1) ** 2 * mesh.nSides * 8  # both slots of every side, 5 doubles per point
     s.close()
-    return {"ratio": t_bwd / t_fwd, "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "adjoint_s": t_bwd,
-            "gradient_total_ratio": (t_fwd_ck + t_bwd) / t_fwd,
-            "memory_bytes": {"checkpoints": ckpt.numel() * 8, "segment_stage_states": every * 5 * state,
-                             "segment_face_states": every * 5 * face_states, "work": 3 * state},
-            "case": "TGV 16^3, N=5 GL split PI Roe, 200 steps, checkpoints every 20 steps"}
+    return {"ratio": t_bwd_tr / t_fwd, "mode": "trajectory stored by the forward (HBM), no recomputation",
+            "forward_s": t_fwd, "forward_with_trajectory_s": t_fwd_tr, "adjoint_s": t_bwd_tr,
+            "gradient_total_ratio": (t_fwd_tr + t_bwd_tr) / t_fwd, "trajectory_bytes": traj_bytes,
+            "gradient_bitwise_equal_to_checkpointed": same,
+            "checkpointed": {"ratio": t_bwd / t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "adjoint_s": t_bwd,
+                             "gradient_total_ratio": (t_fwd_ck + t_bwd) / t_fwd,
+                             "memory_bytes": {"checkpoints": (len(dts) // every) * state,
+                                              "segment_stage_states": every * 5 * state,
+                                              "segment_face_states": every * 5 * face_states, "work": 3 * state}},
+            "case": "TGV 16^3, N=5 GL split PI Roe, 200 steps; checkpointed: checkpoints every 20 steps, "
+                    "segments recomputed"}
 
 
 def curvilinear():

@@ -66,6 +66,7 @@ def lib() -> C.CDLL:
     L.fxg_rhs_vjp_dev.argtypes = [vp, C.c_double, vp, vp, vp, vp]
     L.fxg_forward_n_steps.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, vp, vp, C.c_int, vp]
     L.fxg_adjoint_n_steps.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, vp, vp, vp, C.c_int, vp]
+    L.fxg_adjoint_storage.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]
     L.fxg_pipeline.argtypes = [vp]
     L.fxg_pipeline.restype = C.c_char_p
     L.fxg_sync.argtypes = [vp]
@@ -81,7 +82,8 @@ EXPORTS = ["fxg_last_error", "fxg_version", "fxg_create", "fxg_create_from_nodes
            "fxg_set_state",
            "fxg_get_state", "fxg_get_ut", "fxg_get_gradients", "fxg_get_metrics", "fxg_dg_time_derivative",
            "fxg_timestep_lserkw2", "fxg_calc_timestep", "fxg_dg_time_derivative_dev", "fxg_timestep_lserkw2_dev",
-           "fxg_rhs_vjp_dev", "fxg_forward_n_steps", "fxg_adjoint_n_steps", "fxg_pipeline", "fxg_sync", "fxg_sizes",
+           "fxg_rhs_vjp_dev", "fxg_forward_n_steps", "fxg_adjoint_n_steps", "fxg_adjoint_storage", "fxg_pipeline",
+           "fxg_sync", "fxg_sizes",
            "fxg_profile_enable", "fxg_profile_read"]
 
 KERNEL_NAMES = ["prolong_u", "lift", "flux", "volint", "dt_elem", "dt_reduce", "faces", "visc_faces", "dg_fused"]

@@ -3,7 +3,8 @@
 `lserk_steps(U0, solver, dts)` advances a device state by len(dts) TimeStepByLSERKW2 steps with a
 frozen time-step sequence; its backward runs the library's discrete adjoint
 (`fxg_adjoint_n_steps`: the transposed Jacobian of the right-hand side, stage by stage, with the
-states recomputed from checkpoints). States are torch tensors of shape (nElems, N+1, N+1, N+1, 5),
+states recomputed from checkpoints, or, with checkpoint_every=0, read from the trajectory the
+forward pass stored). States are torch tensors of shape (nElems, N+1, N+1, N+1, 5),
 the memory order of FLEXI's U(5,0:N,0:N,0:N,nElems).
 """
 from __future__ import annotations
@@ -61,8 +62,9 @@ class LSERKSteps(torch.autograd.Function):
         dts = np.asarray(dts, dtype=np.float64)
         every = int(checkpoint_every)
         U = U0.detach().clone().contiguous()
-        # checkpoints of the state every `every` steps, kept for the backward pass
-        ckpt = torch.empty(((len(dts) + every - 1) // every,) + tuple(U.shape), dtype=U.dtype, device=U.device)
+        # checkpoints of the state every `every` steps (every = 0: the whole trajectory of stage
+        # states), kept for the backward pass
+        ckpt = torch.empty(solver.adjoint_storage(len(dts), every), dtype=U.dtype, device=U.device)
         torch.cuda.synchronize()  # the library runs on its own stream
         solver.forward_n_steps_dev(U.data_ptr(), dts, ckpt.data_ptr(), every)
         solver.sync()

@@ -666,15 +666,50 @@ int fxg_rhs_vjp_dev(fxg_ctx* c, double t, const double* U, const double* lam, do
   return 0;
 }
 
+int fxg_adjoint_storage(fxg_ctx* c, int nSteps, int nStages, int checkpointEvery, size_t* doubles) {
+  if (!c || !doubles) return fail("NULL argument");
+  if (nSteps < 0 || nStages < 1 || checkpointEvery < 0) return fail("invalid nSteps, nStages or checkpointEvery");
+  const size_t vol5 = (size_t)5 * c->np * c->K, face10 = (size_t)10 * c->nf * c->S;
+  if (checkpointEvery == 0)
+    *doubles = (size_t)nSteps * nStages * (vol5 + face10);
+  else
+    *doubles = (size_t)((nSteps + checkpointEvery - 1) / checkpointEvery) * vol5;
+  return 0;
+}
+
 int fxg_forward_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, const double* RKA, const double* RKb,
                         const double* RKc, double* U, double* ckpt, int checkpointEvery, void* stream) {
   (void)RKc;
   if (!c || !dt || !U) return fail("NULL argument");
   if (nSteps < 0) return fail("nSteps must not be negative");
-  if (ckpt && checkpointEvery < 1) return fail("checkpointEvery must be positive");
+  if (ckpt && checkpointEvery < 0) return fail("checkpointEvery must not be negative");
   if (check_rk(nStages, RKA, RKb)) return 1;
   const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const size_t vol5 = (size_t)5 * c->np * c->K;
+  if (ckpt && checkpointEvery == 0) {
+    // trajectory: every stage's input state and its face states, written by the stages themselves
+    // (each stage reads state x and writes state x+1 out of place, so storing costs no extra bytes)
+    if (ensure_adjoint(c)) return 1;
+    c->faces_of = nullptr;
+    if (nSteps == 0) return 0;
+    const size_t face10 = (size_t)10 * c->nf * c->S, nTot = (size_t)nSteps * nStages;
+    double* S = ckpt;
+    double* F = ckpt + nTot * vol5;
+    HIPCHECK(hipMemcpyAsync(S, U, vol5 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(fxg::launch_faces_of(c->m, S, F, s));
+    for (size_t x = 0; x < nTot; ++x) {
+      const int it = (int)(x / nStages), st = (int)(x % nStages);
+      const bool last = x + 1 == nTot;
+      fxg::FusedWork fw = c->fw;
+      fw.FU[0] = F + x * face10;
+      fw.FU[1] = last ? c->fw.FU[0] : F + (x + 1) * face10;
+      const fxg::RKStage stage{st == 0 ? 1 : 2, st == 0 ? 0.0 : -RKA[st - 1], RKb[st] * dt[it],
+                               last ? U : S + (x + 1) * vol5, c->R};
+      HIPCHECK(fxg::launch_rhs_fused(c->m, fw, c->gas, c->p.riemann, 0, S + x * vol5, nullptr, stage, true, 0, nullptr,
+                                     s, c->timer));
+    }
+    return 0;
+  }
   for (int it = 0; it < nSteps; ++it) {
     if (ckpt && it % checkpointEvery == 0)
       HIPCHECK(hipMemcpyAsync(ckpt + (size_t)(it / checkpointEvery) * vol5, U, vol5 * sizeof(double),
@@ -694,7 +729,9 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
   (void)RKc;
   if (!c || !dt || !U0 || !ubar) return fail("NULL argument");
   if (nSteps < 0) return fail("nSteps must not be negative");
-  if (checkpointEvery < 1) return fail("checkpointEvery must be positive");
+  if (checkpointEvery < 0) return fail("checkpointEvery must not be negative");
+  if (checkpointEvery == 0 && !ckptIn)
+    return fail("checkpointEvery 0 reads the trajectory stored by fxg_forward_n_steps: ckpt_dev is required");
   if (check_rk(nStages, RKA, RKb)) return 1;
   if (ensure_adjoint(c)) return 1;
   c->faces_of = nullptr;  // the recomputation below runs on scratch states
@@ -702,6 +739,25 @@ int fxg_adjoint_n_steps(fxg_ctx* c, int nSteps, const double* dt, int nStages, c
   const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const size_t vol5 = (size_t)5 * c->np * c->K, bytes = vol5 * sizeof(double);
   const size_t face10 = (size_t)10 * c->nf * c->S;  // both slots of every side, 5 per point
+  if (checkpointEvery == 0) {
+    // the stored trajectory: the reverse stages only, no recomputation
+    const size_t nTot = (size_t)nSteps * nStages;
+    const double* S = ckptIn;
+    const double* F = ckptIn + nTot * vol5;
+    DevBuffers buf;
+    double* rbar = buf.get(vol5);
+    if (!rbar) return fail("fxg_adjoint_n_steps: out of device memory");
+    HIPCHECK(hipMemsetAsync(rbar, 0, bytes, s));
+    fxg::AdjointChain chain;
+    for (size_t x = nTot; x-- > 0;) {
+      const int it = (int)(x / nStages), st = (int)(x % nStages);
+      HIPCHECK(fxg::launch_adjoint_stage(c->m, c->aw, c->gas, c->p.riemann, S + x * vol5, F + x * face10, rbar, ubar,
+                                         RKb[st] * dt[it], st == 0 ? 0.0 : -RKA[st - 1], c->adjTmp, chain, s));
+    }
+    HIPCHECK(fxg::launch_adjoint_flush(c->m, c->aw, rbar, ubar, c->adjTmp, chain, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return 0;
+  }
   const int every = checkpointEvery < nSteps ? checkpointEvery : nSteps;
   const int nck = (nSteps + every - 1) / every;
   DevBuffers buf;

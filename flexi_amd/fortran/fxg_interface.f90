@@ -152,6 +152,13 @@ INTERFACE
     INTEGER(C_INT),VALUE      :: nSteps, nStages, checkpointEvery
     REAL(C_DOUBLE),INTENT(IN) :: dt(*), RKA(*), RKb(*), RKc(*)
   END FUNCTION
+  INTEGER(C_INT) FUNCTION fxg_adjoint_storage(ctx,nSteps,nStages,checkpointEvery,doubles) &
+      BIND(C,NAME='fxg_adjoint_storage')
+    IMPORT :: C_INT, C_SIZE_T, C_PTR
+    TYPE(C_PTR),VALUE              :: ctx
+    INTEGER(C_INT),VALUE           :: nSteps, nStages, checkpointEvery
+    INTEGER(C_SIZE_T),INTENT(OUT)  :: doubles
+  END FUNCTION
   FUNCTION fxg_pipeline(ctx) BIND(C,NAME='fxg_pipeline')
     IMPORT :: C_PTR
     TYPE(C_PTR),VALUE :: ctx

@@ -206,6 +206,14 @@ class DGSolver:
                                                   self._RKc, U0_ptr, ckpt_ptr or None, ubar_ptr,
                                                   int(checkpoint_every), stream or None))
 
+    def adjoint_storage(self, n_steps: int, checkpoint_every: int) -> int:
+        """Doubles of the ckpt buffer for n_steps steps: checkpoints every checkpoint_every steps,
+        or (0) the whole trajectory of stage states that spares the adjoint its recomputation."""
+        n = C.c_size_t()
+        _lib.check(_lib.lib().fxg_adjoint_storage(self.ctx, int(n_steps), self.scheme.nStages, int(checkpoint_every),
+                                                  C.byref(n)))
+        return n.value
+
     def kernel_pipeline(self) -> str:
         """'fused' or 'classic' (which stage kernels this context launches)."""
         return _lib.lib().fxg_pipeline(self.ctx).decode()

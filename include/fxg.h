@@ -152,12 +152,20 @@ int fxg_timestep_lserkw2_dev(fxg_ctx* ctx, double t, double dt, int nStages, con
  *   (host array; the sequence a primal run chose with CALCTIMESTEP, frozen so that the adjoint
  *   differentiates the same map). With ckpt_dev (may be NULL) the state before step k is copied to
  *   ckpt_dev[k/checkpointEvery] for every k that checkpointEvery divides (ceil(nSteps/every) states).
+ *   With checkpointEvery = 0 the forward call instead stores the whole trajectory in ckpt_dev
+ *   (every stage's input state and its face states; layout private to the library, size from
+ *   fxg_adjoint_storage): the stages write it out of place, so the forward costs what it costs
+ *   without it, and the adjoint then needs no recomputation.
  * fxg_adjoint_n_steps: ubar_dev holds dJ/dU(T) on entry and dJ/dU(0) on return, for the map
  *   U(0) -> U(T) of fxg_forward_n_steps started from U0_dev. The stage states are recomputed
  *   segment by segment from the checkpoints: those of a forward call with the same
  *   checkpointEvery (ckpt_dev), or, with ckpt_dev NULL, from a forward sweep of its own. Device
  *   memory allocated for the call: checkpointEvery*nStages + 3 states (+ the checkpoints when it
- *   takes them itself). Synchronous. */
+ *   takes them itself). With checkpointEvery = 0, ckpt_dev must be the trajectory of a forward
+ *   call with checkpointEvery = 0 and the same steps; only the reverse stages run (one state of
+ *   scratch). Synchronous.
+ * fxg_adjoint_storage: the number of doubles ckpt_dev needs for nSteps steps and checkpointEvery
+ *   (ceil(nSteps/checkpointEvery) states, or the trajectory for 0). */
 int fxg_rhs_vjp_dev(fxg_ctx* ctx, double t, const double* U_dev, const double* lam_dev, double* out_dev,
                     void* stream);
 int fxg_forward_n_steps(fxg_ctx* ctx, int nSteps, const double* dt, int nStages, const double* RKA,
@@ -166,6 +174,7 @@ int fxg_forward_n_steps(fxg_ctx* ctx, int nSteps, const double* dt, int nStages,
 int fxg_adjoint_n_steps(fxg_ctx* ctx, int nSteps, const double* dt, int nStages, const double* RKA,
                         const double* RKb, const double* RKc, const double* U0_dev, const double* ckpt_dev,
                         double* ubar_dev, int checkpointEvery, void* stream);
+int fxg_adjoint_storage(fxg_ctx* ctx, int nSteps, int nStages, int checkpointEvery, size_t* doubles);
 /* Which stage kernels the context runs: "fused" (Gauss-Lobatto split form, N 2..7: two
  * kernels per stage, face states carried between LSERK stages) or "classic" (prolongation,
  * lifting, side fluxes, integrals). No counterpart in the reference; FXG_PIPELINE=classic in

@@ -103,3 +103,29 @@ def test_kinetic_energy_gradient_tgv16_n5_200_steps():
     U0 = testcase.tgv(s.metrics()["Elem_xGP"])
     dts = primal_dts(s, U0, 200)
     fd_gradient_check(s, U0, dts, every=20, n_dirs=8, seed=0, tol=1e-5)
+
+
+def test_trajectory_mode_equals_checkpoint_mode_bitwise():
+    """checkpoint_every=0: the forward stores every stage state and its face states, and the
+    adjoint runs the reverse stages without recomputation. The stored stages are the same kernels
+    on the same inputs as the in-place forward and the segment recomputation, so U(T) and the
+    gradient agree bit for bit with the checkpointed run (checkpoints every 3 of 10 steps)."""
+    import torch
+    from flexi_amd.adjoint import kinetic_energy, lserk_steps, node_weights, primal_dts, to_tensor
+    mesh, s = make(5, "roe", shape=(3, 3, 3))
+    U0 = perturbed_tgv(s, 7, amp=1e-3)
+    dts = primal_dts(s, U0, 10)
+    W = node_weights(s)
+    out = {}
+    for every in (3, 0):
+        U0t = to_tensor(U0).requires_grad_(True)
+        UT = lserk_steps(U0t, s, dts, every)
+        kinetic_energy(UT, W).backward()
+        out[every] = (UT.detach().clone(), U0t.grad.detach().clone())
+    n = s.adjoint_storage(10, 0)
+    assert n == 10 * 5 * (5 * 6 ** 3 * mesh.nElems + 10 * 6 ** 2 * mesh.nSides)
+    assert s.adjoint_storage(10, 3) == 4 * 5 * 6 ** 3 * mesh.nElems
+    assert torch.equal(out[0][0], out[3][0])
+    assert torch.equal(out[0][1], out[3][1])
+    assert torch.count_nonzero(out[0][1]).item() > 0.9 * out[0][1].numel()
+    s.close()
