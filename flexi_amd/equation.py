@@ -3,7 +3,8 @@
 ``ref_state_prim`` builds RefStatePrim(6,nRefState) from the parameter file's RefState entries
 (rho, u, v, w, p) as InitEquation does (src/equations/navierstokes/equation.f90:126-141):
 T = p*(1/rho)/R. ``prim_to_cons`` is PrimToCons (idealgas/eos.f90:305-328), the conservative
-state of IniExactFunc = 1 (exactfunc.f90:258-259: RefStateCons of IniRefState).
+state of IniExactFunc = 1 (exactfunc.f90:258-259: RefStateCons of IniRefState);
+``cons_to_prim`` is ConsToPrim (idealgas/eos.f90:203-230).
 """
 from __future__ import annotations
 
@@ -29,3 +30,17 @@ def prim_to_cons(prim: np.ndarray, kappa: float) -> np.ndarray:
     c[3] = prim[3] * prim[0]
     c[4] = (1.0 / (kappa - 1.0)) * prim[4] + 0.5 * (c[1] * prim[1] + c[2] * prim[2] + c[3] * prim[3])
     return c
+
+
+def cons_to_prim(cons: np.ndarray, kappa: float, R: float) -> np.ndarray:
+    """ConsToPrim over the leading axis: cons (5, ...) -> prim (6, ...) = rho, u, v, w, p, T."""
+    c = np.asarray(cons, dtype=np.float64)
+    p = np.empty((6,) + c.shape[1:], order="F")
+    s_rho = 1.0 / c[0]
+    p[0] = c[0]
+    p[1] = c[1] * s_rho
+    p[2] = c[2] * s_rho
+    p[3] = c[3] * s_rho
+    p[4] = (kappa - 1.0) * (c[4] - 0.5 * (c[1] * p[1] + c[2] * p[2] + c[3] * p[3]))
+    p[5] = p[4] * s_rho / R
+    return p

@@ -15,7 +15,7 @@ form), IniExactFunc with IniRefState/RefState (the Taylor-Green vortex of the TG
 uniform RefState for IniExactFunc = 1), kappa, R, Pr, mu0, TStart, TEnd, CFLScale, DFLScale,
 Analyze_dt, nWriteData, maxIter, TimeDiscMethod, nCalcTimeStepMax (only 1), MachNumber and
 IniConstDens (T only) of the TGV testcase, BoundaryName/BoundaryType (boundary types 2 with a
-RefState, 3, 4, 9). The node type is a compile-time choice in FLEXI (FLEXI_NODETYPE); here it is
+RefState, or with BCState 0 and IniExactFunc = 1; 3, 4, 9). The node type is a compile-time choice in FLEXI (FLEXI_NODETYPE); here it is
 ``--node-type``. Other keys are reported and ignored.
 """
 from __future__ import annotations
@@ -113,7 +113,10 @@ class Case:
         if self.analyze_dt <= 0.0:  # analyze.f90:124-130: analyses only at the end
             self.analyze_dt, self.n_write = self.tend, 1
         self.max_iter = get("maxiter", -1, int)
-        self.scheme = SCHEMES[get("timediscmethod", "carpenterrk4-5").lower()]
+        method = get("timediscmethod", "carpenterrk4-5").lower()
+        if method not in SCHEMES:
+            raise ValueError(f"TimeDiscMethod {method}: one of {', '.join(SCHEMES)}")
+        self.scheme = SCHEMES[method]
         if get("ncalctimestepmax", 1, int) != 1:
             raise ValueError("nCalcTimeStepMax other than 1 is not supported")
         self.ini_exact = get("iniexactfunc", 0, int)
@@ -144,6 +147,17 @@ class Case:
 def run(case: Case, out_dir: str, precision: str = "fp64", restart: str | None = None, log=print) -> dict:
     mesh = io.read_mesh_bc(case.mesh_path, case.boundaries)
     ref_prim = equation.ref_state_prim(case.ref_states, case.R) if case.ref_states else None
+    if mesh.nBCSides and np.any((mesh.BoundaryType[:, 0] == 2) & (mesh.BoundaryType[:, 1] == 0)):
+        # Dirichlet boundaries with BCState 0 take ExactFunc(IniExactFunc) as their state
+        # (getboundaryflux.f90:261-265); for IniExactFunc = 1 that is ConsToPrim(RefStateCons(IniRefState)),
+        # passed as one more RefState
+        if case.ini_exact != 1:
+            raise ValueError("Dirichlet boundaries with BCState 0 (exact function) need IniExactFunc = 1")
+        cons = equation.prim_to_cons(ref_prim[:, case.ini_ref - 1], case.kappa)
+        ref_prim = np.asfortranarray(np.column_stack([ref_prim, equation.cons_to_prim(cons, case.kappa, case.R)]))
+        bt = mesh.BoundaryType.copy(order="F")
+        bt[(bt[:, 0] == 2) & (bt[:, 1] == 0), 1] = ref_prim.shape[1]
+        mesh.BoundaryType = bt
     s = DGSolver(mesh, case.N, node_type=case.node_type, split_dg=case.split, riemann=case.riemann,
                  kappa=case.kappa, R=case.R, Pr=case.Pr, mu0=case.mu0, precision=precision, scheme=case.scheme,
                  CFLScale=case.CFL, DFLScale=case.DFL, ref_state_prim=ref_prim)

@@ -255,3 +255,23 @@ def test_gpu_cavity_run_matches_flexis_state(tmp_path):
     print(f"{it} steps: FP64 max |U - U_FLEXI| = {err:.2e}; FP32-bulk rel L2 {rel:.2e}")
     assert it == 353 and err <= 1e-12
     assert rel <= 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_dirichlet_box_freestream_matches_flexis_state(tmp_path):
+    """The reference's timedisc/freestream_3D case (2^3 box, N=3 Gauss, every side a Dirichlet
+    boundary with BCState 0 = the constant exact function, t = 0..1e-6) through the driver with
+    CarpenterRK4-5: the end state equals FLEXI's file for that case (freestream kept to round-off)
+    within 1e-12 absolute."""
+    io = fio()
+    prm = frun.parse_parameter_file(os.path.join(HD, "freestream_3D_parameter.ini"))
+    prm["timediscmethod"] = ["carpenterrk4-5"]  # the regression runs one method per comma entry
+    case = frun.Case(prm, HD, node_type="gauss")
+    res = frun.run(case, str(tmp_path), log=lambda s: None)
+    st = io.read_state(res["states"][-1])
+    ref = io.read_state(os.path.join(HD, "cartbox_State_0000000.000001000.h5"))
+    assert os.path.basename(res["states"][-1]) == "cartbox_State_0000000.000001000.h5"
+    assert st["time"] == ref["time"] == 1e-6
+    err = np.max(np.abs(st["U"] - ref["U"]))
+    print(f"{res['iterations']} steps, max |U - U_FLEXI| = {err:.2e}")
+    assert err <= 1e-12
