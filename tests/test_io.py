@@ -3,9 +3,10 @@
 Fixtures under tests/golden/hdf5/ are unmodified copies of files the reference holds:
 * CART_HEX_PERIODIC_004_mesh.h5 (tutorials/taylorgreenvortex), CART_HEX_PERIODIC_008_mesh.h5
   (regressioncheck/checks/tgv/split): HOPR meshes of the periodic box [0,2pi]^3;
-* cartbox3D_mesh.h5 and cartbox_State_0000000.000000000.h5
-  (regressioncheck/checks/timedisc/freestream_3D): a 2^3 box with Dirichlet boundaries and the
-  state FLEXI wrote at t=0 for RefState (rho,u,v,w,p) = (1,1,1,1,1), N=3 Gauss nodes.
+* cartbox3D_mesh.h5, cartbox_State_0000000.000000000.h5, cartbox_State_0000000.000001000.h5 and
+  freestream_3D_parameter.ini (= parameter.ini there) (regressioncheck/checks/timedisc/
+  freestream_3D): a 2^3 box with Dirichlet boundaries and the states FLEXI wrote at t=0 and
+  t=1e-6 for RefState (rho,u,v,w,p) = (1,1,1,1,1), N=3 Gauss nodes.
 """
 import os
 import re
