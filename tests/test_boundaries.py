@@ -258,14 +258,17 @@ def test_gpu_cavity_run_matches_flexis_state(tmp_path):
 
 
 @pytest.mark.gpu
-def test_gpu_dirichlet_box_freestream_matches_flexis_state(tmp_path):
+@pytest.mark.parametrize("method", ["standardrk3-3", "niegemannrk4-14", "toulorgerk4-8c", "toulorgerk3-7c",
+                                    "toulorgerk4-8f", "carpenterrk4-5"])
+def test_gpu_dirichlet_box_freestream_matches_flexis_state(tmp_path, method):
     """The reference's timedisc/freestream_3D case (2^3 box, N=3 Gauss, every side a Dirichlet
     boundary with BCState 0 = the constant exact function, t = 0..1e-6) through the driver with
-    CarpenterRK4-5: the end state equals FLEXI's file for that case (freestream kept to round-off)
-    within 1e-12 absolute."""
+    each 2N-storage method of its list: the end state equals FLEXI's file for that case
+    (freestream kept to round-off) within 1e-12 absolute."""
     io = fio()
     prm = frun.parse_parameter_file(os.path.join(HD, "freestream_3D_parameter.ini"))
-    prm["timediscmethod"] = ["carpenterrk4-5"]  # the regression runs one method per comma entry
+    assert method in prm["timediscmethod"][0].split(",")
+    prm["timediscmethod"] = [method]  # the regression runs one method per comma entry
     case = frun.Case(prm, HD, node_type="gauss")
     res = frun.run(case, str(tmp_path), log=lambda s: None)
     st = io.read_state(res["states"][-1])

@@ -231,3 +231,37 @@ def test_fused_last_stage_time_step_equals_time_step_kernel():
     assert err1 == err2 == 0
     assert abs(dt1 - dt2) <= 1e-15 * dt2
     np.testing.assert_allclose(e1, e2, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("scheme", ["niegemannrk4-14", "toulorgerk4-8c", "toulorgerk3-7c", "toulorgerk4-8f",
+                                    "standardrk3-3"])
+@pytest.mark.parametrize("N,node,split,riemann", [(5, orc.NODE_GL, True, orc.RIEMANN_ROE),
+                                                  (3, orc.NODE_G, False, orc.RIEMANN_LF)])
+def test_lserk_schemes_match_oracle_stepping(scheme, N, node, split, riemann):
+    """Two steps of each 2N-storage scheme on the GPU (fused path for GL split, four-kernel path
+    for Gauss) against TimeStepByLSERKW2 restated on the oracle's right-hand side
+    (timestep.f90:112-131: R = Ut (stage 1), R = Ut - RKA R; U += RKb dt R)."""
+    from flexi_amd.solver import DGSolver
+    from flexi_amd.timedisc import SCHEMES
+    sc = SCHEMES[scheme]
+    mesh, m, oracle, gpu0, U = setup(N, node, split, riemann)
+    gpu0.close()
+    gpu = DGSolver(mesh, N, node_type=node, split_dg="pi" if split else None, riemann=riemann, metrics=m,
+                   scheme=sc)
+    try:
+        gpu.set_state(U)
+        Uo = np.asfortranarray(U.copy())
+        A = np.concatenate([[0.0], sc.RKA])
+        for _ in range(2):
+            dt, err = gpu.CalcTimeStep()
+            assert err == 0
+            gpu.dt = dt
+            gpu.TimeStepByLSERKW2(0.0)
+            R = np.zeros_like(Uo)
+            for s in range(sc.nStages):
+                R = oracle.rhs(Uo) - A[s] * R
+                Uo = np.asfortranarray(Uo + sc.RKb[s] * dt * R)
+        Ug = gpu.get_state()
+    finally:
+        gpu.close()
+    assert rel(Ug - U, Uo - U) <= 1e-11
