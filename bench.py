@@ -36,11 +36,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet HBM3E bandwidth
 #                + own viscous fluxes written 384*32                              -> 140.25 B/DOF
 #   dg_fused   : U 40 + metrics 80 + neighbour states 30 + neighbour fluxes 24 + U written 40
 #                + next face states 30 + register R read 40 (not in stage 1) + written 40
-#                + MetricsAdv/MetricsVisc 48 (stage 5: CALCTIMESTEP's spectral radii),
-#                averaged over the 5 stages: (284 + 3*324 + 372)/5               -> 325.6 B/DOF
+#                (stage 5 also forms CALCTIMESTEP's spectral radii from the metric terms, no
+#                algorithmic bytes), averaged over the 5 stages: (284 + 4*324)/5  -> 316.0 B/DOF
 # The four-kernel path's figures follow the same counting.
 ALGO_BYTES_PER_DOF = {"prolong_u": 60.0, "lift": 372.0, "flux": 147.0, "volint": 366.0,
-                      "faces": 53.125, "visc_faces": 140.25, "dg_fused": 325.6}
+                      "faces": 53.125, "visc_faces": 140.25, "dg_fused": 316.0}
 # Per DOF-stage byte model of the stage (U, Ut/R, metrics and gradient traffic of one stage)
 B_STAGE = 432.0
 # HBM bytes per DOF of one launch of the fused kernels at this workload (FP64, mean over the 5

@@ -146,7 +146,7 @@ struct fxg_ctx {
   // owned device arrays
   int *E2S = nullptr, *s2v2 = nullptr, *v2s2 = nullptr;
   double *Mf = nullptr, *Mg = nullptr, *Mh = nullptr, *sJ = nullptr, *NV = nullptr, *T1 = nullptr, *T2 = nullptr,
-         *SE = nullptr, *adv = nullptr, *visc = nullptr, *basis = nullptr, *xGP = nullptr;
+         *SE = nullptr, *basis = nullptr, *xGP = nullptr;
   double *dtHost = nullptr;
   int* errHost = nullptr;
   std::vector<void*> owned;
@@ -252,7 +252,8 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const double kappaM1 = p->kappa - 1.0;
   const double cp = p->R * p->kappa / kappaM1;
-  c->gas = fxg::Gas{p->kappa, kappaM1, 1.0 / kappaM1, p->R, 1.0 / p->R, p->mu0, p->mu0 * cp / p->Pr, 4.0 / 3.0, 2.0 / 3.0};
+  c->gas = fxg::Gas{p->kappa, kappaM1, 1.0 / kappaM1, p->R, 1.0 / p->R, p->mu0, p->mu0 * cp / p->Pr, 4.0 / 3.0, 2.0 / 3.0,
+                    std::fmax(4.0 / 3.0, p->kappa / p->Pr)};
   const size_t vol = (size_t)c->np * K, face = (size_t)c->nf * S;
   std::vector<int> s2v2, v2s2;
   build_maps(c->N, s2v2, v2s2);
@@ -262,8 +263,7 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
     return 1;
   if (alloc_owned(c, &c->Mf, 3 * vol) || alloc_owned(c, &c->Mg, 3 * vol) || alloc_owned(c, &c->Mh, 3 * vol) ||
       alloc_owned(c, &c->sJ, vol) || alloc_owned(c, &c->NV, 3 * face) || alloc_owned(c, &c->T1, 3 * face) ||
-      alloc_owned(c, &c->T2, 3 * face) || alloc_owned(c, &c->SE, face) || alloc_owned(c, &c->adv, 3 * vol) ||
-      alloc_owned(c, &c->visc, 3 * vol) || alloc_owned(c, &c->xGP, 3 * vol))
+      alloc_owned(c, &c->T2, 3 * face) || alloc_owned(c, &c->SE, face) || alloc_owned(c, &c->xGP, 3 * vol))
     return 1;
   if (alloc_owned(c, &c->U, 5 * vol) || alloc_owned(c, &c->Ut, 5 * vol) || alloc_owned(c, &c->R, 5 * vol)) return 1;
   HIPCHECK(hipMemsetAsync(c->U, 0, 5 * vol * sizeof(double), c->stream));
@@ -294,14 +294,11 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   HIPCHECK(hipHostMalloc((void**)&c->dtHost, 2 * sizeof(double), hipHostMallocDefault));
   HIPCHECK(hipHostMalloc((void**)&c->errHost, sizeof(int), hipHostMallocDefault));
   c->m = fxg::DevMesh{c->N, K, S, c->E2S, c->s2v2, c->v2s2, c->Mf, c->Mg, c->Mh, c->sJ,
-                      c->NV, c->T1, c->T2, c->SE, c->adv, c->visc, c->basis, 0};
+                      c->NV, c->T1, c->T2, c->SE, c->basis, 0};
   return 0;
 }
 
 int finish_setup(fxg_ctx* c) {
-  const double KappasPr = c->p.kappa / c->p.Pr;
-  const double kmax = std::fmax(4.0 / 3.0, KappasPr);  // KAPPASPR_MAX_TIMESTEP_H, eos.h
-  HIPCHECK(fxg::launch_metrics_adv(c->m, kmax, c->adv, c->visc, c->stream));
   if (c->fused) HIPCHECK(fxg::launch_pack_metrics(c->m, c->fw.Jsoa, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;

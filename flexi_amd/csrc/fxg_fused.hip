@@ -284,8 +284,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
     double lam[6] = {0, 0, 0, 0, 0, 0};
     int isnan_rho = 0;
     if (act)
-      dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, m.MetricsAdv + 3 * gid,
-                     m.MetricsVisc + 3 * gid, lam, isnan_rho);
+      dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
     double mx[6];
     int nanr = 0;
     double* sRd = reinterpret_cast<double*>(sX);

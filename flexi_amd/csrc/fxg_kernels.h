@@ -34,8 +34,6 @@ struct DevMesh {
   const double* TangVec1;
   const double* TangVec2;
   const double* SurfElem;  // (0:N,0:N,nSides)
-  const double* MetricsAdv;   // (3,0:N,0:N,0:N,nElems), calctimestep.f90:42-83
-  const double* MetricsVisc;  // (3,0:N,0:N,0:N,nElems)
   const double* basis;        // BasisLayout
   // Gauss-Lobatto fused stage: 1 = each element takes n*SurfElem of a face from its own metric
   // terms at the face node (metrics computed by the library), 0 = the side's NormVec/SurfElem as
@@ -158,8 +156,6 @@ hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& g
 // CALCTIMESTEP from the spectral-radius sums a fused last stage stored for the current state.
 hipError_t launch_calc_timestep_from_sums(const DevMesh& m, const DevWork& w, const double* lamSum, const int* lamNan,
                                           double CFL, double DFL, hipStream_t s, KernelTimer* timer);
-hipError_t launch_metrics_adv(const DevMesh& m, double KappasPr_max, double* adv, double* visc,
-                              hipStream_t s);
 // Copies a (12, np, nElems) gradient array of type float/double into three FLEXI-layout
 // FP64 arrays (4,0:N,0:N,0:N,nElems).
 hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, double* gx, double* gy,

@@ -565,8 +565,7 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_dt_elem(const double* __restrict
   int isnan_rho = 0;
   if (t < np) {
     const size_t gid = (size_t)np * e + t;
-    dt_node_lambda(U + 5 * gid, gas, m.sJ[gid], m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, m.MetricsAdv + 3 * gid,
-                   m.MetricsVisc + 3 * gid, lam, isnan_rho);
+    dt_node_lambda(U + 5 * gid, gas, m.sJ[gid], m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
   }
   double mx[6];
   int nanr = 0;
@@ -629,21 +628,6 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(const double* __restrict__ d
     out[0] = c;
     out[1] = v;
     errOut[0] = ec;
-  }
-}
-
-// MetricsAdv / MetricsVisc of InitCalctimestep (calctimestep.f90:42-83).
-__global__ __launch_bounds__(256) void k_metrics_adv(DevMesh m, double KappasPr_max, double* __restrict__ adv,
-                                                   double* __restrict__ visc, int nNodes) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= nNodes) return;
-  const double sj = m.sJ[x];
-  const double* M[3] = {m.Mf + 3 * (size_t)x, m.Mg + 3 * (size_t)x, m.Mh + 3 * (size_t)x};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    adv[3 * (size_t)x + d] = sj * sqrt(M[d][0] * M[d][0] + M[d][1] * M[d][1] + M[d][2] * M[d][2]);
-    const double a0 = M[d][0] * sj, a1 = M[d][1] * sj, a2 = M[d][2] * sj;
-    visc[3 * (size_t)x + d] = KappasPr_max * (a0 * a0 + a1 * a1 + a2 * a2);
   }
 }
 
@@ -798,12 +782,6 @@ hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& g
   }
 }
 
-hipError_t launch_metrics_adv(const DevMesh& m, double KappasPr_max, double* adv, double* visc, hipStream_t s) {
-  const int n = m.N + 1;
-  const int nNodes = n * n * n * m.nElems;
-  hipLaunchKernelGGL(k_metrics_adv, dim3((nNodes + 255) / 256), dim3(256), 0, s, m, KappasPr_max, adv, visc, nNodes);
-  return hipGetLastError();
-}
 
 hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, double* gx, double* gy, double* gz,
                                    hipStream_t s) {

@@ -10,6 +10,7 @@ namespace fxg {
 // idealgas/eos.f90:106-196): KappaM1, sKappaM1, cp = R*kappa/(kappa-1), lambda = mu0*cp/Pr.
 struct Gas {
   double kappa, kappaM1, sKappaM1, R, sR, mu0, lambda, s43, s23;
+  double kpmax;  // KAPPASPR_MAX_TIMESTEP_H (eos.h): max(4/3, kappa/Pr)
 };
 
 template <class T>
@@ -187,10 +188,11 @@ __device__ __forceinline__ void riemann_rot(const GasT<T>& g, int solver, bool s
 
 // Per-node spectral radii of CALCTIMESTEP (src/equations/navierstokes/calctimestep.f90:121-190):
 // lam[d] convective = |Ja^d.v sJ| + c*MetricsAdv_d, lam[3+d] viscous = mu/rho*MetricsVisc_d. The
-// time-step kernel and the fused last LSERK stage both use this, so their time steps agree bitwise.
+// time-step kernel and the fused last LSERK stage both use this, with contraction into FMAs off,
+// so their time steps agree bitwise.
 __device__ __forceinline__ void dt_node_lambda(const double* c, const Gas& gas, double sj, const double* M0,
-                                               const double* M1, const double* M2, const double* adv,
-                                               const double* visc, double* lam, int& isnan_rho) {
+                                               const double* M1, const double* M2, double* lam, int& isnan_rho) {
+#pragma clang fp contract(off)
   const double srho = 1.0 / c[0];
   const double vel[3] = {c[1] * srho, c[2] * srho, c[3] * srho};
   const double p = gas.kappaM1 * (c[4] - 0.5 * (vel[0] * c[1] + vel[1] * c[2] + vel[2] * c[3]));
@@ -200,8 +202,13 @@ __device__ __forceinline__ void dt_node_lambda(const double* c, const Gas& gas, 
   const double* M[3] = {M0, M1, M2};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    lam[d] = fabs(M[d][0] * vsJ[0] + M[d][1] * vsJ[1] + M[d][2] * vsJ[2]) + cs * adv[d];
-    lam[3 + d] = gas.mu0 * srho * visc[d];
+    // MetricsAdv and MetricsVisc of InitCalctimestep (calctimestep.f90:42-83), formed here from the
+    // metric terms the spectral radius reads anyway instead of being read as two more arrays
+    const double adv = sj * sqrt(M[d][0] * M[d][0] + M[d][1] * M[d][1] + M[d][2] * M[d][2]);
+    const double a0 = M[d][0] * sj, a1 = M[d][1] * sj, a2 = M[d][2] * sj;
+    const double visc = gas.kpmax * (a0 * a0 + a1 * a1 + a2 * a2);
+    lam[d] = fabs(M[d][0] * vsJ[0] + M[d][1] * vsJ[1] + M[d][2] * vsJ[2]) + cs * adv;
+    lam[3 + d] = gas.mu0 * srho * visc;
   }
 }
 
