@@ -46,7 +46,7 @@ B_STAGE = 432.0
 # HBM bytes per DOF of one launch of the fused kernels at this workload (FP64, mean over the 5
 # stages of a step), from rocprofv3 FETCH_SIZE (x2.0, plain-copy calibration) + WRITE_SIZE runs of
 # this benchmark; the summary and the commands are in profiles/fused_stage_tgv32_n7_fp64.md
-MEASURED_BYTES_PER_DOF = {"dg_fused": 430.7, "visc_faces": 140.6, "faces": 61.3}
+MEASURED_BYTES_PER_DOF = {"dg_fused": 420.0, "visc_faces": 140.6, "faces": 61.3}
 
 
 def make_solver(nel, precision):
