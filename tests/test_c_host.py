@@ -1,7 +1,8 @@
-"""The C ABI used from a plain C program (tests/c_host/fxg_c_host.c): a FLEXI-like driver that
-reads a HOPR mesh, restarts from a FLEXI State file, runs TimeStepByLSERKW2 steps and writes a
-State file, through include/fxg.h and include/fxg_io.h only. On the GPU its result must be
-bit-identical to the same steps driven from Python (ctypes) on the same library."""
+"""The C ABI used from plain C programs, through include/fxg.h and include/fxg_io.h only:
+tests/c_host/fxg_c_host.c, a FLEXI-like driver that reads a HOPR mesh, restarts from a FLEXI
+State file, runs TimeStepByLSERKW2 steps and writes a State file (bit-identical on the GPU to the
+same steps driven from Python), and tests/c_host/fxg_c_cavity.c, FLEXI's cavity_3D regression
+case with boundary conditions and the time loop (FLEXI's own State file at t=1 to 1e-12)."""
 import os
 import shutil
 import subprocess
@@ -15,24 +16,52 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MESH = os.path.join(ROOT, "tests", "golden", "hdf5", "CART_HEX_PERIODIC_004_mesh.h5")
 
 
-def build_host(tmp_path) -> str:
+def build_host(tmp_path, name="fxg_c_host") -> str:
     """gcc against the in-tree libfxg.so and libfxg_io.so (a host program, no GPU code)."""
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
     lib = os.path.join(ROOT, "flexi_amd")
-    exe = str(tmp_path / "fxg_c_host")
+    exe = str(tmp_path / name)
     r = subprocess.run([cc, "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-                        os.path.join(ROOT, "tests", "c_host", "fxg_c_host.c"), "-L", lib, "-lfxg", "-lfxg_io",
-                        f"-Wl,-rpath,{lib}", "-o", exe], capture_output=True, text=True)
+                        os.path.join(ROOT, "tests", "c_host", name + ".c"), "-L", lib, "-lfxg", "-lfxg_io",
+                        f"-Wl,-rpath,{lib}", "-lm", "-o", exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
 
 
-def test_c_host_builds_against_the_headers_and_libraries(tmp_path):
-    exe = build_host(tmp_path)
+@pytest.mark.parametrize("name", ["fxg_c_host", "fxg_c_cavity"])
+def test_c_host_builds_against_the_headers_and_libraries(tmp_path, name):
+    exe = build_host(tmp_path, name)
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+def test_c_cavity_host_reproduces_flexis_regression_state(tmp_path):
+    """FLEXI's parabolic/cavity_3D case from a C program through the C ABI alone (boundary
+    sides, fxg_set_boundaries, CALCTIMESTEP and the dt rules to t=1): FLEXI's State file within
+    its h5diff tolerance (1e-12 absolute), and bit for bit the state the Python driver reaches."""
+    from flexi_amd import io
+    from flexi_amd import run as frun
+    cav = os.path.join(ROOT, "tests", "golden", "cavity")
+    exe = build_host(tmp_path, "fxg_c_cavity")
+    out = str(tmp_path / "cavity_Re100_State_0000001.000000000.h5")
+    print(f"[c host] {exe}", flush=True)
+    r = subprocess.run([exe, os.path.join(cav, "cavity4x4x4_mesh.h5"), out], capture_output=True, text=True,
+                       timeout=200)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout.strip())
+    assert "steps=353" in r.stdout
+    st = io.read_state(out)
+    ref = io.read_state(os.path.join(cav, "reggie_cavity_Re100_State_0000001.000000000.h5"))
+    assert st["time"] == 1.0 and st["node_type"] == "GAUSS"
+    assert np.max(np.abs(st["U"] - ref["U"])) <= 1e-12
+    d = tmp_path / "py"
+    d.mkdir()
+    case = frun.Case(frun.parse_parameter_file(os.path.join(cav, "parameter.ini")), cav, node_type="gauss")
+    res = frun.run(case, str(d), log=lambda s: None)
+    assert np.array_equal(io.read_state(res["states"][-1])["U"], st["U"])
 
 
 @pytest.mark.gpu
