@@ -278,3 +278,47 @@ def test_gpu_dirichlet_box_freestream_matches_flexis_state(tmp_path, method):
     err = np.max(np.abs(st["U"] - ref["U"]))
     print(f"{res['iterations']} steps, max |U - U_FLEXI| = {err:.2e}")
     assert err <= 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_boundary_api_rejects_what_it_does_not_support():
+    """fxg_set_boundaries validates on the host before any kernel runs: unsupported types, a
+    missing RefState index, an exact-function state (0) and a non-positive wall temperature are
+    errors with a message; a mesh with boundary sides refuses a right-hand side until its
+    conditions are set."""
+    import ctypes as C
+    from flexi_amd import _lib
+    from flexi_amd.solver import DGSolver
+    m = cavity_mesh()
+    good = equation.ref_state_prim(CAVITY_REF, 1.0)
+    cases = [
+        ({"BC_free": (23, 1)}, good, "not supported"),
+        ({"BC_free": (2, 0)}, good, "exact function"),
+        ({"BC_free": (2, 3)}, good, "RefState index"),
+        ({}, np.asfortranarray(good * np.array([1, 1, 1, 1, 1, 0.0])[:, None]), "temperature"),
+    ]
+    for over, ref, msg in cases:
+        bt = np.asfortranarray(m.BoundaryType.copy())
+        for name, (ty, st) in over.items():
+            i = m.BCNames.index(name)
+            bt[i, 0], bt[i, 1] = ty, st
+        mm = cavity_mesh()
+        mm.BoundaryType = bt
+        with pytest.raises(_lib.FxgError, match=msg):
+            DGSolver(mm, 2, node_type="gauss", split_dg=None, riemann="roeentropyfix", R=1.0, mu0=0.01,
+                     ref_state_prim=ref)
+    # a context on a mesh with boundary sides and no conditions: the right-hand side is refused
+    s = DGSolver(m, 2, node_type="gauss", split_dg=None, riemann="roeentropyfix", R=1.0, mu0=0.01,
+                 ref_state_prim=good)
+    try:
+        L = _lib.lib()
+        bare = C.c_void_p()
+        assert L.fxg_create_from_nodes(C.byref(s.params), m.nElems, m.nSides,
+                                       np.asfortranarray(m.SideToElem).ctypes.data,
+                                       np.asfortranarray(m.ElemToSide).ctypes.data, m.NGeo,
+                                       np.asfortranarray(m.NodeCoords).ctypes.data, C.byref(bare)) == 0
+        assert L.fxg_dg_time_derivative(bare, 0.0) != 0
+        assert "fxg_set_boundaries" in L.fxg_last_error().decode()
+        L.fxg_destroy(bare)
+    finally:
+        s.close()
