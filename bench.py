@@ -8,11 +8,13 @@ DOF = one volume node (FLEXI's convention); value = DOF * stages * steps / wall 
 
 Also measured (keys of the same JSON line): the FP32-bulk rate and its error against FP64
 ("fp32_bulk"), the adjoint/forward time ratio of the kinetic-energy gradient on TGV 16^3 N=5
-over 200 steps ("adjoint"), and a curvilinear 24^3 N=5 case ("curvilinear").
+over 200 steps ("adjoint"), a curvilinear 24^3 N=5 case ("curvilinear"), a 32^3 N=7 cavity box
+with walls and a Dirichlet lid ("boundaries"), and FLEXI State file write/read rates
+("state_io").
 
 Prints one JSON line. Options: --gpus (only 1 exists on this platform), --steps, --warmup,
 --elems (per direction), --no-cpu (skip the CPU baseline), --no-fp32, --no-adjoint,
---no-curvilinear.
+--no-curvilinear, --no-boundaries, --no-io.
 """
 from __future__ import annotations
 
@@ -190,6 +192,71 @@ def adjoint_ratio():
                     "segments recomputed"}
 
 
+def boundaries(nel=32, steps=20, warmup=3):
+    """Lid-driven cavity box: nel^3 elements on the unit cube, N=7 Gauss-Lobatto split PI, Roe,
+    BR1, isothermal walls and a Dirichlet lid (y+, u=1, lid Mach 0.1), from rest; boundary sides
+    run the four-kernel path. DOF-stage/s, kernel times (the boundary kernels are timed with the
+    lifting and flux kernels they precede/follow) and the roofline of the dominant kernel."""
+    from flexi_amd import equation
+    from flexi_amd.mesh import walled_box
+    from flexi_amd.solver import DGSolver
+    mesh = walled_box(nel, nel, nel, boundary_types={"BC_yplus": (2, 1)})
+    refs = equation.ref_state_prim([[1.0, 1.0, 0.0, 0.0, 71.4285714286], [1.0, 0.0, 0.0, 0.0, 71.4285714286]], 1.0)
+    s = DGSolver(mesh, 7, node_type="gauss-lobatto", split_dg="pi", riemann="roe", R=1.0, mu0=0.01,
+                 ref_state_prim=refs, CFLScale=0.8, DFLScale=0.8)
+    U0 = equation.prim_to_cons(refs[:, 1], 1.4).reshape(5, 1, 1, 1, 1)
+    s.set_state(np.asfortranarray(np.broadcast_to(U0, s.shape)))
+    s.run(warmup)
+    s.sync()
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    kt = s.kernel_times()
+    pipeline = s.kernel_pipeline()
+    s.close()
+    dof = mesh.nElems * 8 ** 3
+    kern = {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in kt.items() if v[1]}
+    main = {k: v for k, v in kern.items() if k in ("prolong_u", "lift", "flux", "volint")}
+    dom = max(main, key=lambda k: main[k]["avg_ms"] * main[k]["launches"])
+    achieved = ALGO_BYTES_PER_DOF[dom] * dof / (main[dom]["avg_ms"] * 1e-3) / 1e9
+    return {"value": dof * 5 * steps / wall, "unit": "DOF-stage/s", "ms_per_step": 1e3 * wall / steps,
+            "pipeline": pipeline, "boundary_sides": mesh.nBCSides, "sides": mesh.nSides, "kernels": kern,
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_dof": ALGO_BYTES_PER_DOF[dom]},
+            "case": f"cavity box {nel}^3, N=7 GL split PI Roe BR1, isothermal walls + Dirichlet lid, {steps} steps"}
+
+
+def state_io(nel=32):
+    """FLEXI State file of the nel^3 N=7 state (5 doubles per DOF) written and read back through
+    libfxg_io.so on the host's /tmp: bytes per second each way (host I/O, not HBM)."""
+    import shutil
+    import tempfile
+    from flexi_amd import io
+    dof = nel ** 3 * 8 ** 3
+    U = np.asfortranarray(np.random.default_rng(0).normal(size=(5, 8, 8, 8, nel ** 3)))
+    d = tempfile.mkdtemp(prefix="fxg_io_")
+    try:
+        p = os.path.join(d, "bench_State_0000000.000000000.h5")
+        t0 = time.perf_counter()
+        io.write_state(p, U, 7, "gauss-lobatto", 0.0, "bench_mesh.h5", project="bench")
+        tw = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        st = io.read_state(p)  # into a fresh array: includes the first touch of its pages
+        tr_fresh = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        st = io.read_state(p, out=st["U"])  # into the host's existing array
+        tr = time.perf_counter() - t0
+        same = bool(np.array_equal(st["U"], U))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    nbytes = 5 * 8 * dof
+    return {"bytes": nbytes, "write_GBps": nbytes / tw / 1e9, "read_GBps": nbytes / tr / 1e9,
+            "read_fresh_array_GBps": nbytes / tr_fresh / 1e9, "round_trip_exact": same,
+            "case": f"DG_Solution of {nel}^3 elements, N=7, written and read with FLEXI's attributes"}
+
+
 def curvilinear():
     """24^3 periodic box mapped by x_i + 0.1 sin x sin y sin z, N=5 Gauss-Lobatto split PI, Roe,
     metrics on the GPU: freestream preservation after 100 steps, then a Gaussian density pulse
@@ -270,6 +337,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-curvilinear", action="store_true")
+    ap.add_argument("--no-boundaries", action="store_true")
+    ap.add_argument("--no-io", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (calibrates HBM counters)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -351,6 +420,12 @@ def main():
     if not args.no_curvilinear:
         print("[bench] curvilinear 24^3 N=5", file=sys.stderr, flush=True)
         out["curvilinear"] = curvilinear()
+    if not args.no_boundaries:
+        print("[bench] boundaries (cavity box 32^3 N=7)", file=sys.stderr, flush=True)
+        out["boundaries"] = boundaries()
+    if not args.no_io:
+        print("[bench] State file I/O", file=sys.stderr, flush=True)
+        out["state_io"] = state_io(args.elems)
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, one step)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.elems)

@@ -117,14 +117,21 @@ def write_state(path: str, U: np.ndarray, N: int, node_type: str, time: float, m
                                     U.shape[4], float(time), _ptr(U, C.c_double)))
 
 
-def read_state(path: str) -> dict:
+def read_state(path: str, out: np.ndarray | None = None) -> dict:
     """A FLEXI State file: {'U': (nVar,N+1,N+1,N+1,nElems) Fortran order, 'N', 'time',
-    'node_type', 'mesh_file'}."""
+    'node_type', 'mesh_file'}. ``out``: an existing float64 Fortran-ordered array of that shape
+    to read into (a host solver's own array; a fresh one costs the first touch of its pages)."""
     L = lib()
     nv, N, K = C.c_int(), C.c_int(), C.c_int()
     _check(L.fxg_io_state_size(path.encode(), C.byref(nv), C.byref(N), C.byref(K)))
     nv, N, K = nv.value, N.value, K.value
-    U = np.zeros((nv, N + 1, N + 1, N + 1, K), dtype=np.float64, order="F")
+    shape = (nv, N + 1, N + 1, N + 1, K)
+    if out is not None:
+        if out.shape != shape or out.dtype != np.float64 or not out.flags.f_contiguous:
+            raise ValueError(f"out must be a Fortran-ordered float64 array of shape {shape}")
+        U = out
+    else:
+        U = np.zeros(shape, dtype=np.float64, order="F")
     t = C.c_double()
     nt = C.create_string_buffer(256)
     mf = C.create_string_buffer(256)

@@ -1,4 +1,4 @@
-"""Periodic hexahedral box meshes in FLEXI's connectivity format.
+"""Hexahedral box meshes in FLEXI's connectivity format (periodic, or with boundary sides).
 
 This is host-side set-up: it produces exactly the arrays FLEXI's mesh reader and
 ``setLocalSideIDs``/``fillMeshInfo`` hand to the DG operator (src/mesh/prepare_mesh.f90:50-737,
@@ -10,7 +10,8 @@ src/mesh/mesh_vars.f90:85-94):
 * ``SideToElem(5,nSides)`` = (ElemID, nbElemID, locSide, nbLocSide, flip),
 
 all 1-based like the Fortran arrays. A periodic box has no boundary sides, so every side is an
-inner side (``firstInnerSide = 1``, ``lastInnerSide = nSides``; src/mesh/mesh.f90:257-268).
+inner side (``firstInnerSide = 1``, ``lastInnerSide = nSides``; src/mesh/mesh.f90:257-268);
+``walled_box`` makes the faces of non-periodic directions boundary sides, numbered first.
 
 Local side ids follow ``flexi.h``: 1 zeta-, 2 eta-, 3 xi+, 4 eta+, 5 xi-, 6 zeta+.
 The master of a side is the element on its minus side (flip 0); the slave's flip is found
@@ -109,6 +110,20 @@ class Mesh:
         return self.nSides
 
 
+def _box_nodes(dims, ijk, lo, length, NGeo, deform) -> np.ndarray:
+    """Equidistant geometry nodes of every element of a box, (3,ng,ng,ng,K) in Fortran order."""
+    h = np.array([length[d] / dims[d] for d in range(3)])
+    xi = np.linspace(0.0, 1.0, NGeo + 1)
+    a = xi[:, None, None, None]
+    b = xi[None, :, None, None]
+    c = xi[None, None, :, None]
+    pts = np.stack(np.broadcast_arrays(lo[0] + (ijk[0] + a) * h[0], lo[1] + (ijk[1] + b) * h[1],
+                                       lo[2] + (ijk[2] + c) * h[2]), axis=-1)  # (ng,ng,ng,K,3)
+    if deform is not None:
+        pts = deform(pts)
+    return np.asfortranarray(np.moveaxis(pts, -1, 0))
+
+
 def periodic_box(nx: int, ny: int, nz: int,
                  lo=(-np.pi, -np.pi, -np.pi), length=(2 * np.pi, 2 * np.pi, 2 * np.pi),
                  NGeo: int = 1,
@@ -125,20 +140,10 @@ def periodic_box(nx: int, ny: int, nz: int,
         raise ValueError("periodic_box needs at least 2 elements per direction")
     K = nx * ny * nz
     dims = (nx, ny, nz)
-    h = np.array([length[d] / dims[d] for d in range(3)])
     e = np.arange(K)
     ijk = np.stack([e % nx, (e // nx) % ny, e // (nx * ny)])  # (3,K)
 
-    # Equidistant geometry nodes of every element, (3,ng,ng,ng,K) in Fortran order.
-    xi = np.linspace(0.0, 1.0, NGeo + 1)
-    a = xi[:, None, None, None]
-    b = xi[None, :, None, None]
-    c = xi[None, None, :, None]
-    pts = np.stack(np.broadcast_arrays(lo[0] + (ijk[0] + a) * h[0], lo[1] + (ijk[1] + b) * h[1],
-                                       lo[2] + (ijk[2] + c) * h[2]), axis=-1)  # (ng,ng,ng,K,3)
-    if deform is not None:
-        pts = deform(pts)
-    NodeCoords = np.asfortranarray(np.moveaxis(pts, -1, 0))
+    NodeCoords = _box_nodes(dims, ijk, lo, length, NGeo, deform)
 
     # Flip of the slave (minus side) relative to the master (plus side), one per direction:
     # match the four face corners on the integer lattice of a 2x2x2 element block.
@@ -186,3 +191,80 @@ def sine_deform(amplitude: float = 0.1) -> Callable[[np.ndarray], np.ndarray]:
         s = amplitude * np.sin(pts[..., 0]) * np.sin(pts[..., 1]) * np.sin(pts[..., 2])
         return pts + s[..., None]
     return f
+
+
+# Boundary indices of a generated box, in HOPR's BCIndex order of a Cartesian box
+# (z-, y-, x+, y+, x-, z+); (direction, sign) -> 1-based index.
+BOX_BC_NAMES = ["BC_zminus", "BC_yminus", "BC_xplus", "BC_yplus", "BC_xminus", "BC_zplus"]
+_BOX_BC_INDEX = {(2, -1): 1, (1, -1): 2, (0, +1): 3, (1, +1): 4, (0, -1): 5, (2, +1): 6}
+
+
+def walled_box(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), length=(1.0, 1.0, 1.0),
+               periodic=(False, False, False), boundary_types=None, NGeo: int = 1) -> Mesh:
+    """Box of ``nx*ny*nz`` hexahedra whose faces in each non-periodic direction are boundary
+    sides. Sides are numbered as setLocalSideIDs does (src/mesh/prepare_mesh.f90:196-243):
+    boundary sides first, in element / local-side order, then the inner sides; a boundary side's
+    element is its master (flip 0) and it has no neighbour (SideToElem(2,:) = -1). ``BC`` holds the
+    boundary index of each boundary side (``BOX_BC_NAMES`` order) and ``BoundaryType(6,3)`` the
+    (type, state, alpha) of each boundary, from ``boundary_types`` ({name: (type, state)}, default
+    (4, 1): isothermal walls at RefState 1)."""
+    for d, n in enumerate((nx, ny, nz)):
+        if n < (2 if periodic[d] else 1):
+            raise ValueError("walled_box needs at least one element per direction (two where periodic)")
+    dims = (nx, ny, nz)
+    K = nx * ny * nz
+    e = np.arange(K)
+    ijk = np.stack([e % nx, (e // nx) % ny, e // (nx * ny)])
+    NodeCoords = _box_nodes(dims, ijk, lo, length, NGeo, None)
+    plus = {0: XI_PLUS, 1: ETA_PLUS, 2: ZETA_PLUS}
+    minus = {0: XI_MINUS, 1: ETA_MINUS, 2: ZETA_MINUS}
+    # the slave flip of an inner side per direction, as periodic_box finds it (sides 1..3 of a
+    # 2x2x2 box are the xi, eta and zeta faces of element 1)
+    flips = {d: int(f) for d, f in enumerate(periodic_box(2, 2, 2).SideToElem[4, :3])}
+    # every local side: inner (master = element on the minus side) or boundary
+    sides = {}  # (e, ls) -> ('bc', index) | ('in', master e, master ls, slave e, slave ls, flip)
+    for el in range(K):
+        for d in range(3):
+            i = ijk[d, el]
+            o = ijk[:, el].copy()
+            if i + 1 < dims[d] or periodic[d]:
+                o[d] = (i + 1) % dims[d]
+                nb = o[0] + nx * (o[1] + ny * o[2])
+                sides[(el, plus[d])] = ("in", el, plus[d], nb, minus[d], flips[d])
+                sides[(nb, minus[d])] = sides[(el, plus[d])]
+            else:
+                sides[(el, plus[d])] = ("bc", _BOX_BC_INDEX[(d, +1)])
+            if i == 0 and not periodic[d]:
+                sides[(el, minus[d])] = ("bc", _BOX_BC_INDEX[(d, -1)])
+    nBC = sum(1 for v in sides.values() if v[0] == "bc")
+    nIn = (len(sides) - nBC) // 2
+    S = nBC + nIn
+    SideToElem = np.full((5, S), -1, dtype=np.int32, order="F")
+    ElemToSide = np.zeros((2, 6, K), dtype=np.int32, order="F")
+    BC = np.zeros(nBC, dtype=np.int32)
+    ids = {}
+    next_bc, next_in = 0, nBC
+    for el in range(K):
+        for ls in range(1, 7):
+            v = sides[(el, ls)]
+            if v[0] == "bc":
+                sid = next_bc
+                next_bc += 1
+                BC[sid] = v[1]
+                SideToElem[0, sid], SideToElem[2, sid] = el + 1, ls
+                ElemToSide[:, ls - 1, el] = (sid + 1, 0)
+                continue
+            if v not in ids:
+                ids[v] = next_in
+                next_in += 1
+            sid = ids[v]
+            _, me, mls, se, sls, fl = v
+            SideToElem[:, sid] = (me + 1, se + 1, mls, sls, fl)
+            ElemToSide[:, ls - 1, el] = (sid + 1, 0 if (el, ls) == (me, mls) else fl)
+    bt = dict(boundary_types or {})
+    BoundaryType = np.zeros((6, 3), dtype=np.int32, order="F")
+    for k, name in enumerate(BOX_BC_NAMES):
+        BoundaryType[k, :2] = bt.get(name, (4, 1))
+    return Mesh(nElems=K, nSides=S, NGeo=NGeo, NodeCoords=NodeCoords, ElemToSide=ElemToSide,
+                SideToElem=SideToElem, shape=dims, box_lo=tuple(lo), box_len=tuple(length), nBCSides=nBC,
+                BC=BC, BoundaryType=BoundaryType, BCNames=list(BOX_BC_NAMES))

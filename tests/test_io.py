@@ -55,6 +55,10 @@ def test_state_round_trip(tmp_path):
     fio.write_state(p, U, 5, "gauss-lobatto", 0.125, "case_mesh.h5", project="case")
     s = fio.read_state(p)
     assert np.array_equal(s["U"], U)
+    buf = np.zeros_like(U, order="F")
+    assert fio.read_state(p, out=buf)["U"] is buf and np.array_equal(buf, U)
+    with pytest.raises(ValueError):
+        fio.read_state(p, out=np.zeros((5, 6, 6, 6, 6), order="F"))
     assert s["N"] == 5 and s["time"] == 0.125
     assert s["node_type"] == "GAUSS-LOBATTO" and s["mesh_file"] == "case_mesh.h5"
     with pytest.raises(ValueError):
