@@ -281,14 +281,18 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
   HIPCHECK(hipMemsetAsync(G, 0, 12 * vol * gsz, c->stream));
   // FXG_PIPELINE=classic keeps the four-kernel path for cases the fused one covers
   const char* pipe = std::getenv("FXG_PIPELINE");
-  c->fused = fxg::fused_supported(c->N, p->nodeType, p->splitDG == FXG_SPLITDG_PI, S) && nBC == 0 &&
+  c->fused = fxg::fused_supported(c->N, p->nodeType, p->splitDG == FXG_SPLITDG_PI, S) &&
              !(pipe && std::strcmp(pipe, "classic") == 0);
   if (c->fused) {
     char* FG;
     if (alloc_owned(c, &c->fw.FU[0], 10 * face) || alloc_owned(c, &c->fw.FU[1], 10 * face) ||
         alloc_owned(c, &FG, 8 * face * gsz) || alloc_owned(c, &c->fw.Jsoa, fxg::packed_metrics_size(c->N, K)) ||
-        alloc_owned(c, &c->lamSum, 2 * (size_t)K) || alloc_owned(c, &c->lamNan, (size_t)K))
+        alloc_owned(c, &c->lamSum, 2 * (size_t)K) || alloc_owned(c, &c->lamNan, (size_t)K) ||
+        alloc_owned(c, &c->fw.bcP, (size_t)c->nf * nBC))
       return 1;
+    char* gBC;
+    if (alloc_owned(c, &gBC, 12 * (size_t)c->nf * nBC * gsz)) return 1;
+    c->fw.gBC = gBC;
     c->fw.FG = FG;
   }
   HIPCHECK(hipHostMalloc((void**)&c->dtHost, 2 * sizeof(double), hipHostMallocDefault));
@@ -363,8 +367,9 @@ int check_rk(int nStages, const double* RKA, const double* RKb) {
 }
 
 int ensure_adjoint(fxg_ctx* c) {
-  if (!c->fused || c->p.precision != FXG_PREC_FP64)
-    return fail("the discrete adjoint is built for the FP64 Gauss-Lobatto split form (N 2..7)");
+  if (!c->fused || c->p.precision != FXG_PREC_FP64 || c->nBC > 0)
+    return fail("the discrete adjoint is built for the FP64 Gauss-Lobatto split form (N 2..7) on meshes without "
+                "boundary sides");
   if (c->aw.FUa) return 0;
   const size_t face = (size_t)c->nf * c->S, vol = (size_t)c->np * c->K;
   if (alloc_owned(c, &c->aw.FUa, 10 * face) || alloc_owned(c, &c->aw.XU[0], 10 * face) ||

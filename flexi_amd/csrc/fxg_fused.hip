@@ -25,6 +25,7 @@
 // round-off, and multiplied by small quantities there). Each element gets FLEXI's contributions:
 // master +Flux, slave -Flux (src/dg/surfint.t90:471-616) with
 // Flux = [F*(U_m,U_s;n_m) + (f_m+f_s).n_m/2]*SurfElem (src/dg/fillflux.f90:40-215).
+#include "fxg_bc.h"
 #include "fxg_elem.h"
 
 namespace fxg {
@@ -45,11 +46,12 @@ struct LdsB {
 // code, which keeps the kernel within 128 VGPRs (two workgroups per CU at N=7) without spills.
 // In FP32-bulk the LDS arrays are FP32 (38 KB at N=7) and the kernel is held to 64 VGPRs, so
 // four workgroups share a CU (six waves per SIMD below N=7, where 8 would spill).
-template <int N, class RF, int SOLVER>
+template <int N, class RF, int SOLVER, bool BC>
 __global__ __launch_bounds__(D<N>::NT)
 __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
                                                      double* __restrict__ FUnext, const RF* __restrict__ FG,
-                                                     Gas gas, double* Ut, RKStage st, RF* __restrict__ Gout) {
+                                                     Gas gas, double* Ut, RKStage st, RF* __restrict__ Gout,
+                                                     const double* __restrict__ bcP) {
   constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
   // LDS holds the working precision RF: the values are rounded to RF where they are stored, as
   // they were rounded where they are read. sA rows: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T, 7 E
@@ -132,8 +134,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
     const RotState<RF> Sn = rotate<RF>((RF)sNb[4 * NF + x], (RF)sNb[6 * NF + x], (RF)sNb[5 * NF + x], (RF)sNb[x],
                                        (RF)sNb[NF + x], (RF)sNb[2 * NF + x], nv, t1, t2);
     RF F[5];
-    if (master) riemann_rot<RF>(gt, SOLVER, true, So, Sn, F);
-    else riemann_rot<RF>(gt, SOLVER, true, Sn, So, F);
+    bool wall = false;
+    if constexpr (BC) {
+      // a wall side: the inviscid flux is (0, p_wall n, 0) (getboundaryflux.f90:580-592)
+      const int side = (int)(own / (2u * (unsigned)nf));
+      if (side < m.nBC && m.bcInfo[2 * side] != BC_DIRICHLET) {
+        wall = true;
+        F[0] = RF(0); F[1] = (RF)bcP[slot_side_point<N>(own)]; F[2] = F[3] = F[4] = RF(0);
+      }
+    }
+    if (!wall) {
+      if (master) riemann_rot<RF>(gt, SOLVER, true, So, Sn, F);
+      else riemann_rot<RF>(gt, SOLVER, true, Sn, So, F);
+    }
     const double lh = minus_side(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N];
     sSurf[x] = ms * (double)F[0] * se * lh;
 #pragma unroll
@@ -339,9 +352,14 @@ hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int rie
     Scope sc(tm, K_FACES, s);
     hipLaunchKernelGGL((k_faces_of<N>), dim3(K), dim3(NT), 0, s, U, m, w.FU[cur]);
   }
+  const bool bc = m.nBC > 0;
   {
     Scope sc(tm, K_VISC_FACES, s);
-    const hipError_t err = launch_visc_faces(m, sizeof(RF) == 4, U, w.Jsoa, w.FU[cur], w.FG, gas, s);
+    if (bc) {
+      const hipError_t err = launch_bc_ghost(m, w.FU[cur], w.bcP, gas, s);
+      if (err != hipSuccess) return err;
+    }
+    const hipError_t err = launch_visc_faces(m, sizeof(RF) == 4, U, w.Jsoa, w.FU[cur], w.FG, gas, w.gBC, s);
     if (err != hipSuccess) return err;
   }
   {
@@ -349,12 +367,17 @@ hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int rie
     const double* FUc = w.FU[cur];
     double* FUn = w.FU[1 - cur];
     const RF* FG = (const RF*)w.FG;
-    if (riemann == 1)
-      hipLaunchKernelGGL((k_dg_fused<N, RF, 1>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, (RF*)G);
-    else if (riemann == 3)
-      hipLaunchKernelGGL((k_dg_fused<N, RF, 3>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, (RF*)G);
-    else
-      hipLaunchKernelGGL((k_dg_fused<N, RF, 33>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, (RF*)G);
+#define FXG_LAUNCH(SOLV, BCV)                                                                                   \
+  hipLaunchKernelGGL((k_dg_fused<N, RF, SOLV, BCV>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, \
+                     (RF*)G, w.bcP)
+    if (riemann == 1) {
+      if (bc) FXG_LAUNCH(1, true); else FXG_LAUNCH(1, false);
+    } else if (riemann == 3) {
+      if (bc) FXG_LAUNCH(3, true); else FXG_LAUNCH(3, false);
+    } else {
+      if (bc) FXG_LAUNCH(33, true); else FXG_LAUNCH(33, false);
+    }
+#undef FXG_LAUNCH
   }
   return hipGetLastError();
 }

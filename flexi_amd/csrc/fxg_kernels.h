@@ -98,6 +98,8 @@ struct FusedWork {
   double* FU[2];  // face states, 5 per side point and slot, double-buffered across stages
   void* FG;       // viscous normal face fluxes f_v.(n SurfElem), 4 per side point and slot
   double* Jsoa;   // metric terms of the boundary nodes, rows of NB per element: Ja^1..3 x,y,z, sJ
+  double* bcP = nullptr;  // wall pressure (PRESSURE_RIEMANN) per boundary side point, (0:N,0:N,nBC)
+  void* gBC = nullptr;    // lifted gradients at the boundary side points, 12 per point (gradient type)
 };
 // Doubles of FusedWork::Jsoa.
 size_t packed_metrics_size(int N, int nElems);
@@ -111,7 +113,11 @@ bool fused_supported(int N, int nodeType, int split, long long nSides);
 // G (optional, (np,12,nElems) of the gradient type) receives the lifted gradients.
 // Kernel A of the fused stage (fxg_visc.hip): viscous normal face fluxes into FG (float if fp32).
 hipError_t launch_visc_faces(const DevMesh& m, bool fp32, const double* U, const double* Jsoa, const double* FU,
-                             void* FG, const Gas& gas, hipStream_t s);
+                             void* FG, const Gas& gas, void* gBC, hipStream_t s);
+// Boundary sides on the fused path (fxg_visc.hip): the outer slot of each boundary side point gets a
+// face state whose BR1 lifting difference with the inner state is the boundary lifting flux (and,
+// for Dirichlet sides, whose Riemann flux is the boundary flux); bcP the walls' pressure.
+hipError_t launch_bc_ghost(const DevMesh& m, double* FU, double* bcP, const Gas& gas, hipStream_t s);
 hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, int fp32,
                             const double* U, double* Ut, const RKStage& st, bool faces_fresh, int cur,
                             void* G, hipStream_t s, KernelTimer* timer);

@@ -102,8 +102,9 @@ int fxg_destroy(fxg_ctx* ctx);
  * (mesh_vars.f90), BoundaryType(nBCs,3) = (type, state, alpha) in Fortran layout, RefStatePrim(6,
  * nRefState) = (rho,u,v,w,p,T) per RefState (equation.f90:126-141). Types: 2 Dirichlet with
  * state = RefState index (>= 1), 3 adiabatic wall, 4 isothermal wall (temperature of RefState
- * `state`), 9 slip wall (getboundaryflux.f90:259-465, 480-789, 855-939). Meshes with boundary
- * sides run the four-kernel path. No-op for a mesh without boundary sides. */
+ * `state`), 9 slip wall (getboundaryflux.f90:259-465, 480-789, 855-939). Both stage pipelines
+ * support them; the discrete adjoint does not. Call it before the first right-hand side, with no
+ * work of the context in flight. No-op for a mesh without boundary sides. */
 int fxg_set_boundaries(fxg_ctx* ctx, int nBCs, const int* BC, const int* BoundaryType, int nRefState,
                        const double* RefStatePrim);
 

@@ -207,8 +207,9 @@ BC_SETS = {
 @pytest.mark.parametrize("bcs", list(BC_SETS))
 @pytest.mark.parametrize("N,node,split,riemann", CASES)
 def test_gpu_rhs_with_boundaries_matches_oracle(N, node, split, riemann, bcs):
-    """Right-hand side and lifted gradients of the four-kernel GPU path on the cavity mesh with
-    every supported boundary type, against the oracle, for a smooth non-trivial state."""
+    """Right-hand side and lifted gradients on the cavity mesh with every supported boundary type,
+    against the oracle, for a smooth non-trivial state: the four-kernel path (Gauss, weak GL) and
+    the fused path (GL split form, boundary sides through the outer face slots)."""
     m = cavity_mesh(BC_SETS[bcs])
     onode = orc.NODE_G if node == "gauss" else orc.NODE_GL
     oriem = {"lf": orc.RIEMANN_LF, "roe": orc.RIEMANN_ROE, "roeentropyfix": orc.RIEMANN_ROEENTROPYFIX}[riemann]
@@ -218,7 +219,7 @@ def test_gpu_rhs_with_boundaries_matches_oracle(N, node, split, riemann, bcs):
     gref = S.gradients()
     g = gpu_solver(m, N, node, split, riemann)
     try:
-        assert g.kernel_pipeline() == "classic"
+        assert g.kernel_pipeline() == ("fused" if split else "classic")
         out = g.rhs(U)
         gg = g.gradients()
     finally:
@@ -345,3 +346,39 @@ def test_generated_walled_box_equals_the_hopr_cavity_mesh():
     err = np.max(np.abs(Ur - Ub[..., perm])) / np.max(np.abs(Ub))
     print(f"max |Ut(HOPR) - Ut(walled_box)| / max |Ut| = {err:.2e}")
     assert err <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bcs", list(BC_SETS))
+def test_gpu_fused_boundary_steps_match_four_kernel_path(bcs, monkeypatch):
+    """Ten CALCTIMESTEP + LSERK steps of the GL split form (N=5, Roe) on the cavity mesh with
+    boundary sides: the fused path against the four-kernel path (FXG_PIPELINE=classic), and the
+    fused FP32-bulk path against both."""
+    m = cavity_mesh(BC_SETS[bcs])
+    S, met = oracle_solver(m, 5, orc.NODE_GL, 1, orc.RIEMANN_ROE)
+    U0 = smooth_state(met["Elem_xGP"], seed=9)
+    out = {}
+    for name, env, prec in (("fused", None, "fp64"), ("classic", "classic", "fp64"), ("fused32", None, "fp32-bulk"),
+                            ("classic32", "classic", "fp32-bulk")):
+        if env:
+            monkeypatch.setenv("FXG_PIPELINE", env)
+        else:
+            monkeypatch.delenv("FXG_PIPELINE", raising=False)
+        g = gpu_solver(m, 5, "gauss-lobatto", "pi", "roe", precision=prec)
+        try:
+            assert g.kernel_pipeline() == ("classic" if env else "fused")
+            g.set_state(U0)
+            t = g.run(10)
+            out[name] = (g.get_state(), t)
+        finally:
+            g.close()
+    (Uf, tf), (Uc, tc), (U32, _), (Uc32, _) = out["fused"], out["classic"], out["fused32"], out["classic32"]
+    err = np.linalg.norm(Uf - Uc) / np.linalg.norm(Uc - U0)
+    err32 = np.linalg.norm(U32 - Uf) / np.linalg.norm(Uf)
+    errc32 = np.linalg.norm(Uc32 - Uc) / np.linalg.norm(Uc)
+    print(f"{bcs}: fused vs classic {err:.2e} (of U-U0), dt sum {tf:.6e} vs {tc:.6e}; fp32-bulk vs fp64 rel "
+          f"{err32:.2e} fused, {errc32:.2e} four-kernel")
+    assert abs(tf - tc) <= 1e-13 * tc and err <= 1e-11
+    # this state's pressure varies by 5 % around 71: FP32 round-off of the pressure terms dominates;
+    # the fused boundary path must not add to what the four-kernel FP32 path shows
+    assert err32 <= 2.0 * errc32 + 1e-7
