@@ -63,7 +63,7 @@ class DGSolver:
         else:
             keep = {k: _f(metrics[k]) for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ",
                                                "NormVec", "TangVec1", "TangVec2", "SurfElem")}
-            m = _lib.FxgMesh(mesh.nElems, mesh.nSides, 1, mesh.nSides, _ptr(s2e), _ptr(e2s),
+            m = _lib.FxgMesh(mesh.nElems, mesh.nSides, mesh.firstInnerSide, mesh.lastInnerSide, _ptr(s2e), _ptr(e2s),
                              *[_ptr(keep[k]) for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ",
                                                        "NormVec", "TangVec1", "TangVec2", "SurfElem")])
             _lib.check(L.fxg_create(C.byref(self.params), C.byref(m), C.byref(self.ctx)))

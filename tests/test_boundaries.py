@@ -382,3 +382,25 @@ def test_gpu_fused_boundary_steps_match_four_kernel_path(bcs, monkeypatch):
     # this state's pressure varies by 5 % around 71: FP32 round-off of the pressure terms dominates;
     # the fused boundary path must not add to what the four-kernel FP32 path shows
     assert err32 <= 2.0 * errc32 + 1e-7
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,node,split", [(3, "gauss", None), (5, "gauss-lobatto", "pi")])
+def test_gpu_boundaries_with_host_metrics(N, node, split):
+    """fxg_create with the host's metric arrays (firstInnerSide = nBCSides + 1): the same
+    right-hand side as the oracle on those metrics, on both pipelines."""
+    from flexi_amd.solver import DGSolver
+    m = cavity_mesh(BC_SETS["adiabatic+slip"])
+    onode = orc.NODE_G if node == "gauss" else orc.NODE_GL
+    S, met = oracle_solver(m, N, onode, 1 if split else 0, orc.RIEMANN_ROEENTROPYFIX)
+    U = smooth_state(met["Elem_xGP"], seed=2)
+    ref = S.rhs(U)
+    g = DGSolver(m, N, node_type=node, split_dg=split, riemann="roeentropyfix", R=1.0, mu0=0.01, metrics=met,
+                 ref_state_prim=equation.ref_state_prim(CAVITY_REF, 1.0))
+    try:
+        out = g.rhs(U)
+    finally:
+        g.close()
+    err = np.max(np.abs(out - ref)) / np.max(np.abs(ref))
+    print(f"N={N} {node}: {err:.2e}")
+    assert err <= 1e-12
