@@ -145,7 +145,8 @@ int fxg_dg_time_derivative_dev(fxg_ctx* ctx, double t, const double* U_dev, doub
 int fxg_timestep_lserkw2_dev(fxg_ctx* ctx, double t, double dt, int nStages, const double* RKA,
                              const double* RKb, const double* RKc, double* U_dev, double* R_dev,
                              void* stream);
-/* Discrete adjoint (FP64, Gauss-Lobatto split form, N 2..7; no counterpart in the reference,
+/* Discrete adjoint (FP64, Gauss-Lobatto split form, N 2..7, meshes without boundary sides; no
+ * counterpart in the reference,
  * which has no adjoint). All arrays in FLEXI layout on the context's device.
  * fxg_rhs_vjp_dev: out = (d Ut / d U)^T lam, the transposed Jacobian of
  *   DGTimeDerivative_weakForm at U applied to lam (asynchronous on `stream`, NULL = the context's).
