@@ -146,8 +146,8 @@ int fxg_timestep_lserkw2_dev(fxg_ctx* ctx, double t, double dt, int nStages, con
                              const double* RKb, const double* RKc, double* U_dev, double* R_dev,
                              void* stream);
 /* Discrete adjoint (FP64, Gauss-Lobatto split form, N 2..7, meshes without boundary sides; no
- * counterpart in the reference,
- * which has no adjoint). All arrays in FLEXI layout on the context's device.
+ * counterpart in the reference, which has no adjoint). All arrays in FLEXI layout on the
+ * context's device.
  * fxg_rhs_vjp_dev: out = (d Ut / d U)^T lam, the transposed Jacobian of
  *   DGTimeDerivative_weakForm at U applied to lam (asynchronous on `stream`, NULL = the context's).
  * fxg_forward_n_steps: nSteps calls of TimeStepByLSERKW2 on U_dev with the time steps dt[0..nSteps-1]
