@@ -13,10 +13,12 @@ analysis intervals and at the end (TEnd, or maxIter iterations), named
 Keys read: ProjectName, MeshFile, N, Riemann (LF, Roe, RoeEntropyFix), SplitDG (PI; absent: weak
 form), IniExactFunc with IniRefState/RefState (the Taylor-Green vortex of the TGV testcase, or a
 uniform RefState for IniExactFunc = 1), kappa, R, Pr, mu0, TStart, TEnd, CFLScale, DFLScale,
-Analyze_dt, nWriteData, maxIter, TimeDiscMethod, nCalcTimeStepMax (only 1), MachNumber and
-IniConstDens (T only) of the TGV testcase, BoundaryName/BoundaryType (boundary types 2 with a
-RefState, or with BCState 0 and IniExactFunc = 1; 3, 4, 9). The node type is a compile-time choice in FLEXI (FLEXI_NODETYPE); here it is
-``--node-type``. Other keys are reported and ignored.
+Analyze_dt, nWriteData, maxIter, TimeDiscMethod, nCalcTimeStepMax (only 1), MachNumber,
+IniConstDens (T only), NAnalyze and nAnalyzeTestCase of the TGV testcase (whose analysis goes to
+``<ProjectName>_TGVAnalysis.csv`` as FLEXI writes it), BoundaryName/BoundaryType (boundary types
+2 with a RefState, or with BCState 0 and IniExactFunc = 1; 3, 4, 9). The node type is a
+compile-time choice in FLEXI (FLEXI_NODETYPE); here it is ``--node-type``. Other keys are
+reported and ignored.
 """
 from __future__ import annotations
 
@@ -28,13 +30,14 @@ import time
 
 import numpy as np
 
-from . import equation, io, testcase
+from . import analysis, equation, io, testcase
 from .solver import DGSolver
 from .timedisc import SCHEMES
 
 USED = {"projectname", "meshfile", "n", "riemann", "splitdg", "iniexactfunc", "inirefstate", "refstate", "kappa",
         "r", "pr", "mu0", "tstart", "tend", "cflscale", "dflscale", "analyze_dt", "nwritedata", "maxiter",
-        "timediscmethod", "ncalctimestepmax", "machnumber", "iniconstdens", "boundaryname", "boundarytype"}
+        "timediscmethod", "ncalctimestepmax", "machnumber", "iniconstdens", "boundaryname", "boundarytype",
+        "nanalyze", "nanalyzetestcase", "nwritestats"}
 
 
 def parse_parameter_file(path: str) -> dict:
@@ -123,6 +126,8 @@ class Case:
         if self.ini_exact not in (0, 1):
             raise ValueError("IniExactFunc: only 0 (the Taylor-Green vortex testcase) and 1 (RefState)")
         self.mach = get("machnumber", 0.1, _real)  # TGV testcase, testcase.f90:61
+        self.n_analyze_points = get("nanalyze", 2 * (self.N + 1), int)  # analyze.f90:118
+        self.n_analyze_testcase = get("nanalyzetestcase", 10, int)  # testcase.f90:59
         if get("iniconstdens", "T").upper().strip(".") not in ("T", "TRUE"):
             raise ValueError("IniConstDens = F (constant temperature TGV) is not supported")
         self.ref_states = [_array(v) for v in prm.get("refstate", [])]
@@ -187,6 +192,20 @@ def _time_loop(case: Case, s: DGSolver, n_elems: int, out_dir: str, restart: str
         written.append(p)
         return p
 
+    # the TGV testcase's analysis file (InitTestcase, testcase.f90:132-162) and its first line,
+    # from the right-hand side TimeDisc evaluates before the loop (timedisc.f90:136,157)
+    tgv = None
+    if case.ini_exact == 0:
+        sJ = s.metrics()["sJ"]
+        tgv = analysis.TGVAnalysisFile(os.path.join(out_dir, f"{case.project}_TGVAnalysis.csv"),
+                                       keep=restart is not None)
+
+        def analyze_tgv(tt):
+            gx, gy, gz = s.gradients()
+            tgv.append(tt, analysis.tgv_analyze(s.get_state(), gx, gy, gz, sJ, case.N, s.node_type, case.kappa,
+                                                case.R, case.mu0, case.n_analyze_points))
+        s.DGTimeDerivative_weakForm(t)
+        analyze_tgv(t)
     # TimeDisc (timedisc.f90:100-212): the state at the start time is written, also on restart
     write(t)
     tend, t_analyze = case.tend, min(t + case.analyze_dt, case.tend)
@@ -208,6 +227,12 @@ def _time_loop(case: Case, s: DGSolver, n_elems: int, out_dir: str, restart: str
         if it == case.max_iter:
             tend = t_analyze = t
             do_analyze = do_finalize = True
+        if do_analyze:  # a fresh right-hand side for the analysis (timedisc_func.f90:324-325)
+            s.DGTimeDerivative_weakForm(t)
+        if tgv is not None and (it % case.n_analyze_testcase == 0 or do_analyze):
+            # AnalyzeTestcase (timedisc_func.f90:340): between analyses with the gradients of the
+            # last stage's right-hand side, as FLEXI's lifted gradients are then
+            analyze_tgv(t)
         if do_analyze:
             s.sync()
             wall = time.perf_counter() - t_wall

@@ -34,7 +34,7 @@ def test_reads_the_tgv_regression_parameter_file():
     assert c.ref_states == [[1.0, 1.0, 0.0, 0.0, 17194.8345650329]]
     assert c.scheme.nStages == 5  # CarpenterRK4-5, FLEXI's default TimeDiscMethod
     # keys of the file the driver does not act on are reported, none of the ones it needs
-    assert "nanalyze" in c.ignored and "eddyvisctype" in c.ignored
+    assert (c.n_analyze_points, c.n_analyze_testcase) == (10, 10) and "eddyvisctype" in c.ignored
     assert not set(c.ignored) & frun.USED
 
 
@@ -142,3 +142,28 @@ def test_driver_runs_the_tgv_case_and_restarts_bitwise(tmp_path):
         assert np.array_equal(sc["U"], ref.get_state())
     finally:
         ref.close()
+
+
+@pytest.mark.gpu
+def test_driver_writes_flexis_tgv_analysis_file(tmp_path):
+    """The reference's tgv/split case through the driver on the GPU for 21 iterations: the
+    TGVAnalysis.csv lines at iterations 0, 10 and 20 (analysis every nAnalyzeTestCase = 10 steps,
+    between analyses with the last stage's gradients as in FLEXI) equal the reference CSV's first
+    three lines to 1e-9 (the regression check itself uses 1e-4), columns that vanish compared
+    absolutely; the file has FLEXI's header."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import flexi_fixtures as fx
+    header, data = fx.tgv_split_csv()
+    res = frun.run(tgv_case(max_iter=21), str(tmp_path), log=lambda s: None)
+    assert res["iterations"] == 21
+    path = tmp_path / "TGV_Re1600_Split_TGVAnalysis.csv"
+    lines = path.read_text().splitlines()
+    assert lines[0] == ",".join(header)
+    got = np.array([[float(v) for v in ln.split(",")] for ln in lines[1:]])
+    assert got.shape == (4, 16)  # iterations 0, 10, 20 and the last one, 21
+    for r in range(3):
+        for c in range(16):
+            ref = data[r, c]
+            tol = 1e-12 if abs(ref) < 1e-6 else 1e-9 * abs(ref)
+            assert abs(got[r, c] - ref) <= tol, (r, header[c], got[r, c], ref)
