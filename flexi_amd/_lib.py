@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
     L.fxg_get_state.argtypes = [vp, vp]
     L.fxg_get_ut.argtypes = [vp, vp]
     L.fxg_get_gradients.argtypes = [vp, vp, vp, vp]
+    L.fxg_tgv_analyze.argtypes = [vp, C.c_int, vp]
     L.fxg_get_metrics.argtypes = [vp] + [vp] * 9
     L.fxg_dg_time_derivative.argtypes = [vp, C.c_double]
     L.fxg_timestep_lserkw2.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, dp, dp]
@@ -80,7 +81,8 @@ def lib() -> C.CDLL:
 # Every exported symbol of include/fxg.h (checked by the CPU test suite).
 EXPORTS = ["fxg_last_error", "fxg_version", "fxg_create", "fxg_create_from_nodes", "fxg_destroy", "fxg_set_boundaries",
            "fxg_set_state",
-           "fxg_get_state", "fxg_get_ut", "fxg_get_gradients", "fxg_get_metrics", "fxg_dg_time_derivative",
+           "fxg_get_state", "fxg_get_ut", "fxg_get_gradients", "fxg_tgv_analyze", "fxg_get_metrics",
+           "fxg_dg_time_derivative",
            "fxg_timestep_lserkw2", "fxg_calc_timestep", "fxg_dg_time_derivative_dev", "fxg_timestep_lserkw2_dev",
            "fxg_rhs_vjp_dev", "fxg_forward_n_steps", "fxg_adjoint_n_steps", "fxg_adjoint_storage", "fxg_pipeline",
            "fxg_sync", "fxg_sizes",

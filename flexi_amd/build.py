@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfxg.so")
-SOURCES = ["fxg_kernels.hip", "fxg_fused.hip", "fxg_visc.hip", "fxg_adjoint.hip", "fxg_metrics.hip", "fxg_api.hip"]
+SOURCES = ["fxg_kernels.hip", "fxg_fused.hip", "fxg_visc.hip", "fxg_adjoint.hip", "fxg_metrics.hip", "fxg_analyze.hip", "fxg_api.hip"]
 ARCH = os.environ.get("FXG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

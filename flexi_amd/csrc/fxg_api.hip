@@ -164,6 +164,10 @@ struct fxg_ctx {
   bool rhs_stale = false;
   double last_b_dt = 0.0;
   double* Uprev = nullptr;
+  // testcase analysis: per-element partial sums, the interpolation tables and the 13 sums
+  double* anaPart = nullptr;
+  double* anaTab = nullptr;
+  double* anaOut = nullptr;
   // CALCTIMESTEP spectral-radius sums per element, computed by a fused last stage for lam_of
   double* lamSum = nullptr;
   int* lamNan = nullptr;
@@ -564,6 +568,45 @@ int fxg_get_gradients(fxg_ctx* c, double* gx, double* gy, double* gz) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return fail(std::string("fxg_get_gradients: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int fxg_tgv_analyze(fxg_ctx* c, int NAnalyze, double* out) {
+  if (!c || !out) return fail("NULL argument");
+  if (NAnalyze < 1 || NAnalyze > fxg::TGV_NANALYZE_MAX)
+    return fail("fxg_tgv_analyze: NAnalyze must be in 1.." + std::to_string(fxg::TGV_NANALYZE_MAX));
+  if (refresh_last_rhs(c)) return 1;
+  const int n = c->n, na = NAnalyze + 1, nt = (fxg::TGV_NANALYZE_MAX + 1) * (n + 1) + n;
+  if (!c->anaPart && (alloc_owned(c, &c->anaPart, (size_t)fxg::TGV_NSUMS * c->K) ||
+                      alloc_owned(c, &c->anaTab, (size_t)nt) || alloc_owned(c, &c->anaOut, (size_t)fxg::TGV_NSUMS)))
+    return 1;
+  // V(na, n): Lagrange polynomials of the solution nodes at the NAnalyze Gauss-Lobatto points
+  // (Vdm_GaussN_NAnalyze, analyze.f90:258-264), then their weights and the solution weights
+  std::vector<double> x(n), w(n), wb(n), xa(na), wa(na), L(n), tab((size_t)na * n + na + n);
+  fxg::basis::nodes(c->N, c->p.nodeType == FXG_NODE_GAUSS ? fxg::basis::GAUSS : fxg::basis::GAUSS_LOBATTO, x.data(),
+                    w.data());
+  fxg::basis::bary_weights(c->N, x.data(), wb.data());
+  fxg::basis::nodes(NAnalyze, fxg::basis::GAUSS_LOBATTO, xa.data(), wa.data());
+  for (int a = 0; a < na; ++a) {
+    fxg::basis::lagrange_at(xa[a], c->N, x.data(), wb.data(), L.data());
+    for (int i = 0; i < n; ++i) tab[(size_t)a * n + i] = L[i];
+    tab[(size_t)na * n + a] = wa[a];
+  }
+  for (int i = 0; i < n; ++i) tab[(size_t)na * n + na + i] = w[i];
+  double sums[fxg::TGV_NSUMS];
+  HIPCHECK(hipMemcpyAsync(c->anaTab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(fxg::launch_tgv_analyze(c->m, c->U, c->w.G, c->p.precision == FXG_PREC_FP32_BULK, c->anaTab, na, c->gas,
+                                   c->anaPart, c->anaOut, c->stream));
+  HIPCHECK(hipMemcpyAsync(sums, c->anaOut, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  // normalisation (testcase.f90:443-459), rho0 = 1 (testcase_vars.f90:35)
+  const double rho0 = 1.0, Vol = sums[12], mu0 = c->gas.mu0;
+  const double Ekin = sums[2] / Vol, DR_S = sums[6] * 2.0 * mu0 / (rho0 * Vol),
+               DR_Sd = sums[7] * 2.0 * mu0 / (rho0 * Vol), DR_p = -sums[8] / (rho0 * Vol);
+  const double v[15] = {DR_S, DR_Sd + DR_p, Ekin, sums[3] / Vol / rho0, sums[4] / (rho0 * Vol), sums[5] * mu0 / Vol,
+                        DR_S, DR_Sd, DR_p, sums[11], sums[0] / Vol, std::sqrt(2.0 / 3.0 * Ekin), sums[1] / Vol,
+                        sums[9] / Vol, sums[10] * 4.0 / 3.0 / Vol};
+  for (int q = 0; q < 15; ++q) out[q] = v[q];
   return 0;
 }
 

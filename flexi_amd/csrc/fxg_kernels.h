@@ -166,6 +166,16 @@ hipError_t launch_calc_timestep_from_sums(const DevMesh& m, const DevWork& w, co
 // FP64 arrays (4,0:N,0:N,0:N,nElems).
 hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, double* gx, double* gy,
                                    double* gz, hipStream_t s);
+
+// Taylor-Green vortex analysis (AnalyzeTestcase): per-element partial sums part(13, nElems) of
+// the integrals at na^3 Gauss-Lobatto points, then their sum (maximum for the vorticity) over
+// the elements into out[13]: T, entropy, Ekin, Ekin_comp, enstrophy, DR_u, DR_S, DR_Sd, DR_p,
+// ED_S, ED_D (unnormalised), max |vorticity|, domain volume. tab holds V(na, n) row-major
+// (Lagrange polynomials of the solution nodes at the analysis points), wA(na), wGP(n).
+constexpr int TGV_NANALYZE_MAX = 31;
+constexpr int TGV_NSUMS = 13;
+hipError_t launch_tgv_analyze(const DevMesh& m, const double* U, const void* G, int fp32, const double* tab, int na,
+                              const Gas& gas, double* part, double* out, hipStream_t s);
 bool supported_degree(int N);
 
 }  // namespace fxg

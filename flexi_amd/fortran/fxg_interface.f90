@@ -34,6 +34,7 @@ END TYPE fxg_mesh
 
 PUBLIC :: fxg_last_error, fxg_version, fxg_create, fxg_create_from_nodes, fxg_destroy
 PUBLIC :: fxg_set_state, fxg_get_state, fxg_get_ut, fxg_get_gradients, fxg_get_metrics
+PUBLIC :: fxg_tgv_analyze
 PUBLIC :: fxg_dg_time_derivative, fxg_timestep_lserkw2, fxg_calc_timestep
 PUBLIC :: fxg_dg_time_derivative_dev, fxg_timestep_lserkw2_dev, fxg_pipeline, fxg_sync, fxg_sizes
 PUBLIC :: fxg_rhs_vjp_dev, fxg_forward_n_steps, fxg_adjoint_n_steps
@@ -94,6 +95,13 @@ INTERFACE
     IMPORT :: C_INT, C_DOUBLE, C_PTR
     TYPE(C_PTR),VALUE          :: ctx
     REAL(C_DOUBLE),INTENT(OUT) :: gx(*), gy(*), gz(*)
+  END FUNCTION
+  ! the 15 values of writeBuf in AnalyzeTestcase (testcase.f90:464-466), evaluated on the device
+  INTEGER(C_INT) FUNCTION fxg_tgv_analyze(ctx,NAnalyze,vals) BIND(C,NAME='fxg_tgv_analyze')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR),VALUE          :: ctx
+    INTEGER(C_INT),VALUE       :: NAnalyze
+    REAL(C_DOUBLE),INTENT(OUT) :: vals(15)
   END FUNCTION
   INTEGER(C_INT) FUNCTION fxg_get_metrics(ctx,Mf,Mg,Mh,sJ,NormVec,TangVec1,TangVec2,SurfElem,Elem_xGP) &
       BIND(C,NAME='fxg_get_metrics')

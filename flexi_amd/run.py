@@ -196,14 +196,11 @@ def _time_loop(case: Case, s: DGSolver, n_elems: int, out_dir: str, restart: str
     # from the right-hand side TimeDisc evaluates before the loop (timedisc.f90:136,157)
     tgv = None
     if case.ini_exact == 0:
-        sJ = s.metrics()["sJ"]
         tgv = analysis.TGVAnalysisFile(os.path.join(out_dir, f"{case.project}_TGVAnalysis.csv"),
                                        keep=restart is not None)
 
         def analyze_tgv(tt):
-            gx, gy, gz = s.gradients()
-            tgv.append(tt, analysis.tgv_analyze(s.get_state(), gx, gy, gz, sJ, case.N, s.node_type, case.kappa,
-                                                case.R, case.mu0, case.n_analyze_points))
+            tgv.append(tt, s.tgv_analyze(case.n_analyze_points))
         s.DGTimeDerivative_weakForm(t)
         analyze_tgv(t)
     # TimeDisc (timedisc.f90:100-212): the state at the start time is written, also on restart

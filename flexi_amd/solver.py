@@ -121,6 +121,15 @@ class DGSolver:
         _lib.check(_lib.lib().fxg_get_gradients(self.ctx, *[_ptr(x) for x in g]))
         return g
 
+    def tgv_analyze(self, n_analyze: int) -> np.ndarray:
+        """The 15 values of one TGVAnalysis line (AnalyzeTestcase,
+        src/testcase/taylorgreenvortex/testcase.f90:241-479), evaluated on the device from the
+        state and the gradients ``gradients()`` would return, on n_analyze+1 Gauss-Lobatto
+        points per direction."""
+        out = np.zeros(15)
+        _lib.check(_lib.lib().fxg_tgv_analyze(self.ctx, int(n_analyze), _ptr(out)))
+        return out
+
     def metrics(self) -> dict:
         n, K, S = self.N + 1, self.mesh.nElems, self.mesh.nSides
         out = dict(Metrics_fTilde=np.zeros((3, n, n, n, K), order="F"), Metrics_gTilde=np.zeros((3, n, n, n, K), order="F"),

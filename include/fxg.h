@@ -118,6 +118,14 @@ int fxg_get_ut(fxg_ctx* ctx, double* Ut);
  * last stage's right-hand side (at the state before the last update; the fused Gauss-Lobatto
  * path recomputes them there on request, equal to round-off). fxg_get_ut likewise. */
 int fxg_get_gradients(fxg_ctx* ctx, double* gx, double* gy, double* gz);
+/* Taylor-Green vortex analysis, AnalyzeTestcase (src/testcase/taylorgreenvortex/testcase.f90:241-479)
+ * evaluated on the device: the state and the lifted velocity gradients that fxg_get_gradients
+ * would return are interpolated to NAnalyze+1 Gauss-Lobatto points per direction
+ * (1 <= NAnalyze <= 31) and integrated; out[15] receives the values of one TGVAnalysis line in
+ * FLEXI's order (DR_S, DR_Sd+DR_p, Ekin, Ekin comp, Enstrophy, DR_u, DR_S, DR_Sd, DR_p, maximum
+ * vorticity, mean temperature, uprime, mean entropy, ED_S, ED_D), constant viscosity, rho0 = 1.
+ * Replaces the body of AnalyzeTestcase on a GPU host (no download of U and the gradients). */
+int fxg_tgv_analyze(fxg_ctx* ctx, int NAnalyze, double* out);
 /* Geometry as used on the device (for checking fxg_create_from_nodes): same layouts as fxg_mesh
  * plus Elem_xGP(3,0:N,0:N,0:N,nElems). Any pointer may be NULL. */
 int fxg_get_metrics(fxg_ctx* ctx, double* Mf, double* Mg, double* Mh, double* sJ,
