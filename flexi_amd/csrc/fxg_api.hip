@@ -308,6 +308,12 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
 
 int finish_setup(fxg_ctx* c) {
   if (c->fused) HIPCHECK(fxg::launch_pack_metrics(c->m, c->fw.Jsoa, c->stream));
+  if (c->fused) {  // the fused kernels read the side slots of their face points from this table
+    unsigned* slot = nullptr;
+    if (alloc_owned(c, &slot, (size_t)6 * c->nf * c->K)) return 1;
+    HIPCHECK(fxg::launch_slot_table(c->m, slot, c->stream));
+    c->m.slotTab = slot;
+  }
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -62,6 +62,12 @@ __device__ __forceinline__ unsigned face_slot(const DevMesh& m, int e, int x) {
   const int* pq = m.v2s2 + 2 * (ab + nf * (flip + 5 * (ls - 1)));
   return (unsigned)(((size_t)side * 2 + (flip ? 1 : 0)) * nf + pq[0] + n * pq[1]);
 }
+// The same from the table of the fused path (m.slotTab, filled by launch_slot_table): one load
+// instead of ElemToSide followed by the inverse S2V2 map.
+template <int N>
+__device__ __forceinline__ unsigned face_slot_tab(const DevMesh& m, int e, int x) {
+  return m.slotTab[(size_t)6 * (N + 1) * (N + 1) * e + x];
+}
 template <int N>
 __device__ __forceinline__ bool slot_is_master(unsigned own) { return ((own / (unsigned)((N + 1) * (N + 1))) & 1u) == 0; }
 template <int N>

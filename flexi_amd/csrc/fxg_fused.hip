@@ -316,7 +316,7 @@ __global__ __launch_bounds__(D<N>::NT) void k_faces_of(const double* __restrict_
   constexpr int n = D<N>::n, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
   __shared__ unsigned sIdx[NF];
   const int e = blockIdx.x, t = threadIdx.x;
-  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot<N>(m, e, x);
+  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot_tab<N>(m, e, x);
   __syncthreads();
   if (t >= np) return;
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
@@ -404,7 +404,30 @@ __global__ __launch_bounds__(256) void k_pack_metrics(DevMesh m, size_t nNodes, 
   Je[9 * NB] = m.sJ[x];
 }
 
+template <int N>
+__global__ __launch_bounds__(256) void k_slot_table(DevMesh m, size_t total, unsigned* __restrict__ slot) {
+  constexpr int NF = 6 * (N + 1) * (N + 1);
+  const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const int e = (int)(g / NF), x = (int)(g - (size_t)e * NF);
+  slot[g] = face_slot<N>(m, e, x);
+}
+
 }  // namespace
+
+hipError_t launch_slot_table(const DevMesh& m, unsigned* slot, hipStream_t s) {
+  if (m.slotTab) return hipErrorInvalidValue;
+  const size_t total = (size_t)6 * (m.N + 1) * (m.N + 1) * m.nElems;
+  const dim3 g((unsigned)((total + 255) / 256)), b(256);
+  switch (m.N) {
+#define FXG_CASE(NN) \
+  case NN: hipLaunchKernelGGL((k_slot_table<NN>), g, b, 0, s, m, total, slot); break;
+    FXG_CASE(2) FXG_CASE(3) FXG_CASE(4) FXG_CASE(5) FXG_CASE(6) FXG_CASE(7)
+#undef FXG_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_faces_of(const DevMesh& m, const double* U, double* FU, hipStream_t s) {
   switch (m.N) {

@@ -44,6 +44,9 @@ struct DevMesh {
   int nBC = 0;
   const int* bcInfo = nullptr;
   const double* refPrim = nullptr;
+  // face_slot of every own face point of every element, (6*(N+1)^2, nElems), or null (then
+  // formed from ElemToSide and the inverse S2V2 map)
+  const unsigned* slotTab = nullptr;
 };
 
 // Work arrays of one right-hand side evaluation. Face arrays of U are FP64 (U(5,p,q,side));
@@ -107,6 +110,8 @@ size_t packed_metrics_size(int N, int nElems);
 hipError_t launch_faces_of(const DevMesh& m, const double* U, double* FU, hipStream_t s);
 // Fills FusedWork::Jsoa (boundary-node metric terms) from the FLEXI-layout metric arrays of `m`.
 hipError_t launch_pack_metrics(const DevMesh& m, double* Jsoa, hipStream_t s);
+// Fills slot(6*(N+1)^2, nElems) with face_slot of every own face point (m.slotTab must be null).
+hipError_t launch_slot_table(const DevMesh& m, unsigned* slot, hipStream_t s);
 bool fused_supported(int N, int nodeType, int split, long long nSides);
 // One stage of the fused path. FU[cur] must hold the face states of U unless faces_fresh is
 // false, in which case they are rebuilt first. In RK modes the new state's faces go to FU[1-cur].

@@ -35,7 +35,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : (N == 7 
   }
   for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
   for (int x = t; x < NF; x += NT) {
-    const unsigned own = face_slot<N>(m, e, x);
+    const unsigned own = face_slot_tab<N>(m, e, x);
     sIdx[x] = own;
     const Prim q = cons_to_prim(FU + 5 * (size_t)slot_nbr<N>(own), gas);
     sNb[x] = q.u; sNb[NF + x] = q.v; sNb[2 * NF + x] = q.w; sNb[3 * NF + x] = q.T;
