@@ -177,9 +177,11 @@ def test_driver_runs_the_whole_tgv_split_regression_case(tmp_path):
     DR_p at t = 0 is the round-off zero of an integral that vanishes for the divergence-free
     initial state (5.6e-18 in the reference), compared absolutely; the Maximum Vorticity, a
     pointwise maximum, exceeds 1e-4 on three lines near t = 12.4, where it rises by 14 % in 0.05
-    time units (at most 2.4e-4 measured), and is bounded at 3e-4 on at most five lines. The two
-    pipelines of the library differ from each other by a fifteenth of that on the same case
-    (different operation order; the difference grows from round-off)."""
+    time units (at most 2.4e-4 measured), and is bounded at 3e-4 on at most five lines. The gap
+    grows from round-off once the vortex starts its transition near t = 5. The library's two
+    pipelines differ from each other by about 1/15 of it (their operation orders differ). The
+    oracle, which rounds as FLEXI's build does (no FMA), differs from the reference by 6.7e-5 in
+    that column over the same 5527 steps."""
     import sys
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     import flexi_fixtures as fx
