@@ -1,5 +1,7 @@
 """Pin the oracle's DG operator where FLEXI's fixtures do not reach: closed forms, invariants and
 the reference's own Taylor-Green regression trajectory (regressioncheck/checks/tgv/split)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -193,3 +195,46 @@ def test_oracle_reproduces_flexi_tgv_trajectory():
             row = data[it // 10]
             assert abs(t - row[0]) <= 1e-13 * row[0]
             check(row, S.tgv_analyze(U, S.gradients()))
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(os.environ.get("FXG_FULL_TGV") != "1",
+                    reason="about 23 minutes on 8 CPU threads; opt in with FXG_FULL_TGV=1")
+def test_oracle_whole_tgv_split_case_within_flexis_tolerance():
+    """The whole regressioncheck tgv/split case (5527 CarpenterRK4-5 steps to t = 13 with
+    UpdateTimeStep's dt rules, analysis every 10 iterations and at the end after a fresh right-hand
+    side) against all 554 lines of the reference CSV at the check's own tolerance, 1e-4 relative
+    per value; DR_p at t = 0, the round-off zero of an integral that vanishes for the initial
+    state, compared absolutely. Measured: maximum vorticity 6.7e-5, dissipation rates 3.9e-6,
+    time 4.0e-7."""
+    from flexi_amd import run as frun
+    header, data = fx.tgv_split_csv()
+    mesh = periodic_box(8, 8, 8, lo=(0, 0, 0))
+    prm = orc.params(7, orc.NODE_GL, 1, orc.RIEMANN_ROEENTROPYFIX, **PRM)
+    m = orc.calc_metrics(7, orc.NODE_GL, mesh)
+    S = orc.Solver(prm, mesh, m)
+    U = testcase.tgv(m["Elem_xGP"])
+    S.rhs(U)
+    cfl, dfl = orc.fill_cfl_dfl(7, orc.NODE_GL, 0.8, 0.8)
+    rows = [[0.0, *S.tgv_analyze(U, S.gradients())]]
+    t, R, it = 0.0, orc.F(U.shape), 0
+    while True:
+        dt_min, err, _ = S.calc_timestep(U, cfl, dfl)
+        assert err == 0
+        dt, do_analyze, do_finalize = frun.update_time_step(dt_min, t, 13.0, 13.0)
+        S.step(U, t, dt, R)
+        t += dt
+        it += 1
+        if do_analyze:
+            S.rhs(U)
+        if it % 10 == 0 or do_analyze:
+            rows.append([t, *S.tgv_analyze(U, S.gradients())])
+        if do_finalize:
+            break
+    got = np.array(rows)
+    assert it == 5527 and got.shape == data.shape
+    ip = header.index("DR_p")
+    assert abs(got[0, ip] - data[0, ip]) <= 1e-16
+    rel = np.abs(got - data) / np.maximum(np.abs(data), 1e-300)
+    rel[0, ip] = 0.0
+    assert rel.max() <= 1e-4, np.argwhere(rel > 1e-4)
