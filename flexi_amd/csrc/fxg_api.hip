@@ -630,7 +630,7 @@ int fxg_get_metrics(fxg_ctx* c, double* Mf, double* Mg, double* Mh, double* sJ, 
 }
 
 int fxg_dg_time_derivative(fxg_ctx* c, double t) {
-  (void)t;  // no time-dependent sources or boundaries on periodic meshes
+  (void)t;  // no source terms, and the boundary states (RefStates, walls) do not depend on time
   if (!c) return fail("ctx is NULL");
   fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
   c->rhs_stale = false;
@@ -642,7 +642,7 @@ int fxg_dg_time_derivative(fxg_ctx* c, double t) {
 int fxg_timestep_lserkw2(fxg_ctx* c, double t, double dt, int nStages, const double* RKA, const double* RKb,
                          const double* RKc) {
   (void)t;
-  (void)RKc;  // stage times only enter time-dependent sources, which periodic flows do not have
+  (void)RKc;  // stage times only enter time-dependent sources or boundary states, which this path has none of
   if (!c) return fail("ctx is NULL");
   c->lam_of = nullptr;
   if (lserk(c, dt, nStages, RKA, RKb, c->U, c->R, c->stream, c->fused && c->faces_of == c->U, true)) return 1;
