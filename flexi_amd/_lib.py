@@ -2,13 +2,19 @@
 
 The library must have been built in-tree (``python -m flexi_amd.build`` or
 ``__graft_entry__.build()``); a missing library is an error, there is no CPU fallback.
+``FXG_LIB=strict`` in the environment loads ``libfxg_strict.so`` instead: the same kernels
+compiled without FMA contraction, so that every multiply and add rounds as written.
 """
 from __future__ import annotations
 
 import ctypes as C
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfxg.so")
+_VARIANT = os.environ.get("FXG_LIB", "default")
+if _VARIANT not in ("default", "strict"):
+    raise ImportError(f"FXG_LIB={_VARIANT!r}: expected 'default' or 'strict'")
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "libfxg.so" if _VARIANT == "default" else "libfxg_strict.so")
 
 
 class FxgParams(C.Structure):
@@ -56,7 +62,7 @@ def lib() -> C.CDLL:
     L.fxg_set_state.argtypes = [vp, vp]
     L.fxg_get_state.argtypes = [vp, vp]
     L.fxg_get_ut.argtypes = [vp, vp]
-    L.fxg_get_gradients.argtypes = [vp, vp, vp, vp]
+    L.fxg_get_gradients.argtypes = [vp, C.c_int, vp, vp, vp]
     L.fxg_tgv_analyze.argtypes = [vp, C.c_int, vp]
     L.fxg_get_metrics.argtypes = [vp] + [vp] * 9
     L.fxg_dg_time_derivative.argtypes = [vp, C.c_double]

@@ -54,11 +54,25 @@ def primal_dts(solver: DGSolver, U0: np.ndarray, n_steps: int) -> np.ndarray:
     return dts
 
 
+def _check_state(name: str, T: torch.Tensor, solver: DGSolver) -> None:
+    """The library reads and writes a device array of (nElems, N+1, N+1, N+1, 5) doubles through
+    the pointer; anything else would be addressed out of bounds."""
+    n = solver.N + 1
+    shape = (solver.mesh.nElems, n, n, n, 5)
+    if T.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64, got {T.dtype}")
+    if not T.is_cuda:
+        raise ValueError(f"{name} must be a device (cuda) tensor")
+    if tuple(T.shape) != shape:
+        raise ValueError(f"{name} must have shape {shape}, got {tuple(T.shape)}")
+
+
 class LSERKSteps(torch.autograd.Function):
     """U(T) = LSERK^n(U0) with frozen time steps; backward by the discrete adjoint."""
 
     @staticmethod
     def forward(ctx, U0: torch.Tensor, solver: DGSolver, dts, checkpoint_every: int):
+        _check_state("U0", U0, solver)
         dts = np.asarray(dts, dtype=np.float64)
         every = int(checkpoint_every)
         U = U0.detach().clone().contiguous()
@@ -75,6 +89,7 @@ class LSERKSteps(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gU: torch.Tensor):
         U0, ckpt = ctx.saved_tensors
+        _check_state("grad of U(T)", gU, ctx.solver)
         ubar = gU.detach().clone().contiguous()
         torch.cuda.synchronize()
         ctx.solver.adjoint_n_steps_dev(U0.data_ptr(), ubar.data_ptr(), ctx.dts, ctx.every, ckpt.data_ptr())

@@ -3,6 +3,11 @@
 ``python -m flexi_amd.build`` or ``flexi_amd.build.build()``. Every source is compiled on each call
 (the check that the library builds); objects go to ``flexi_amd/_build`` and the shared library to
 ``flexi_amd/libfxg.so``.
+
+A second library, ``flexi_amd/libfxg_strict.so``, is the same code compiled with
+``-ffp-contract=off``: no multiply and add is fused into one FMA, so every operation rounds as
+written, as in FLEXI's own build of the Fortran. The loader picks it with ``FXG_LIB=strict``
+(``flexi_amd/_lib.py``); it exists for parity runs whose outcome depends on that rounding.
 """
 from __future__ import annotations
 
@@ -28,9 +33,14 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+# library variants: file name -> extra compile flags
+VARIANTS = {"libfxg.so": [], "libfxg_strict.so": ["-ffp-contract=off"]}
+
+
+def _compile(src: str, variant: str = "libfxg.so") -> str:
+    odir = BUILD if variant == "libfxg.so" else os.path.join(BUILD, os.path.splitext(variant)[0])
+    obj = os.path.join(odir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC, *FLAGS, *VARIANTS[variant], "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-8000:]}")
@@ -62,18 +72,24 @@ def build_io(verbose: bool = False) -> str | None:
 
 
 def build(verbose: bool = False) -> str:
+    """Builds libfxg_io.so and every library variant; returns the path of libfxg.so."""
     build_io(verbose)
-    os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")
-    os.replace(tmp, LIB)
-    if verbose:
-        print("built", LIB)
+    jobs = [(src, v) for v in VARIANTS for src in SOURCES]
+    for v in VARIANTS:
+        os.makedirs(BUILD if v == "libfxg.so" else os.path.join(BUILD, os.path.splitext(v)[0]), exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 8)) as ex:
+        objs = list(ex.map(lambda j: _compile(*j), jobs))
+    for v in VARIANTS:
+        lib = os.path.join(HERE, v)
+        tmp = lib + ".tmp"
+        mine = [o for o, (_, jv) in zip(objs, jobs) if jv == v]
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *mine]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")
+        os.replace(tmp, lib)
+        if verbose:
+            print("built", lib)
     return LIB
 
 

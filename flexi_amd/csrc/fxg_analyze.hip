@@ -16,6 +16,7 @@
 #include "fxg_physics.h"
 
 namespace fxg {
+
 namespace {
 
 constexpr int NF = 15;  // interpolated fields: U(5), GradVel(3,3) column by column, sJ

@@ -158,13 +158,22 @@ struct fxg_ctx {
   // the state whose face values FU[cur] holds (only trusted for the context's own state, which
   // changes only through fxg_set_state and the library's own updates)
   const double* faces_of = nullptr;
-  // After a fused LSERK step on the context's state, Ut and the lifted gradients of its last
-  // right-hand side (at the state before the last update, U - RKb_last*dt*R) are recomputed on
-  // request: the fused stages do not store them.
+  // Ut and the lifted gradients (G) of the context's last right-hand side. After an LSERK step on
+  // the context's state that is the last stage's, at the state before the last update,
+  // U - RKb_last*dt*R: the fused stages do not store Ut or the gradients, the four-kernel stages
+  // not Ut, so they are recomputed there on request (rhs_stale). The device-pointer calls
+  // overwrite G with the gradients of the caller's state and also mark them stale.
   bool rhs_stale = false;
+  int last_rhs = 0;  // 0: none since creation or fxg_set_state; 1: a right-hand side of U; 2: an LSERK step of U
   double last_b_dt = 0.0;
   double* Uprev = nullptr;
+  // the state whose lifted gradients G holds (c->U or Uprev), null when unknown
+  const double* grads_state = nullptr;
+  // fxg_get_gradients: one direction in FLEXI's layout (5 per node) and the density gradient
+  double* gradOut = nullptr;
+  double* rhoG = nullptr;
   // testcase analysis: per-element partial sums, the interpolation tables and the 13 sums
+  // (allocated at the first fxg_tgv_analyze)
   double* anaPart = nullptr;
   double* anaTab = nullptr;
   double* anaOut = nullptr;
@@ -269,7 +278,10 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
       alloc_owned(c, &c->sJ, vol) || alloc_owned(c, &c->NV, 3 * face) || alloc_owned(c, &c->T1, 3 * face) ||
       alloc_owned(c, &c->T2, 3 * face) || alloc_owned(c, &c->SE, face) || alloc_owned(c, &c->xGP, 3 * vol))
     return 1;
-  if (alloc_owned(c, &c->U, 5 * vol) || alloc_owned(c, &c->Ut, 5 * vol) || alloc_owned(c, &c->R, 5 * vol)) return 1;
+  // Uprev: the last stage's input state, formed when Ut or the gradients of a step are requested
+  if (alloc_owned(c, &c->U, 5 * vol) || alloc_owned(c, &c->Ut, 5 * vol) || alloc_owned(c, &c->R, 5 * vol) ||
+      alloc_owned(c, &c->Uprev, 5 * vol))
+    return 1;
   HIPCHECK(hipMemsetAsync(c->U, 0, 5 * vol * sizeof(double), c->stream));
   HIPCHECK(hipMemsetAsync(c->Ut, 0, 5 * vol * sizeof(double), c->stream));
   HIPCHECK(hipMemsetAsync(c->R, 0, 5 * vol * sizeof(double), c->stream));
@@ -403,18 +415,23 @@ struct DevBuffers {
   }
 };
 
-// Ut and gradients of the last right-hand side of a fused LSERK step on the context's state:
-// the stage input was U_{s-1} = U - RKb_s*dt*R (to round-off), and one right-hand side there
-// restores what the four-kernel path leaves in Ut and the gradient array.
+// Ut and gradients of the context's last right-hand side when they are not in Ut and G
+// (rhs_stale): after an LSERK step the last stage's input was U_{s-1} = U - RKb_s*dt*R (to
+// round-off), and one right-hand side there restores them; after a right-hand side of U, or with
+// none yet, the right-hand side of U.
 int refresh_last_rhs(fxg_ctx* c) {
   if (!c->rhs_stale) return 0;
-  const size_t vol5 = (size_t)5 * c->np * c->K;
-  if (!c->Uprev && alloc_owned(c, &c->Uprev, vol5)) return 1;
-  HIPCHECK(hipMemcpyAsync(c->Uprev, c->U, vol5 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  HIPCHECK(fxg::launch_axpy(vol5, -c->last_b_dt, c->R, c->Uprev, c->stream));
+  const double* X = c->U;
+  if (c->last_rhs == 2) {
+    const size_t vol5 = (size_t)5 * c->np * c->K;
+    HIPCHECK(hipMemcpyAsync(c->Uprev, c->U, vol5 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(fxg::launch_axpy(vol5, -c->last_b_dt, c->R, c->Uprev, c->stream));
+    X = c->Uprev;
+  }
   fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
-  if (rhs(c, c->Uprev, c->Ut, st, c->stream)) return 1;
+  if (rhs(c, X, c->Ut, st, c->stream)) return 1;
   c->rhs_stale = false;
+  c->grads_state = X;
   return 0;
 }
 
@@ -539,6 +556,8 @@ int fxg_set_state(fxg_ctx* c, const double* U) {
   if (!c || !U) return fail("NULL argument");
   c->faces_of = nullptr;
   c->rhs_stale = false;
+  c->last_rhs = 0;
+  c->grads_state = nullptr;
   c->lam_of = nullptr;
   HIPCHECK(hipMemcpyAsync(c->U, U, 5 * (size_t)c->np * c->K * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -560,50 +579,62 @@ int fxg_get_ut(fxg_ctx* c, double* Ut) {
   return 0;
 }
 
-int fxg_get_gradients(fxg_ctx* c, double* gx, double* gy, double* gz) {
+int fxg_get_gradients(fxg_ctx* c, int nVarLifting, double* gx, double* gy, double* gz) {
   if (!c || !gx || !gy || !gz) return fail("NULL argument");
+  if (nVarLifting != 4 && nVarLifting != 5)
+    return fail("fxg_get_gradients: nVarLifting must be 4 (PP_OPTLIFT=1) or 5 (PP_OPTLIFT=0)");
   if (refresh_last_rhs(c)) return 1;
-  const size_t nn = 4 * (size_t)c->np * c->K;
-  double* tmp = nullptr;
-  if (dev_alloc(&tmp, 3 * nn)) return 1;
-  hipError_t e = fxg::launch_unpack_gradients(c->m, c->w.G, c->p.precision == FXG_PREC_FP32_BULK, tmp, tmp + nn,
-                                              tmp + 2 * nn, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(gx, tmp, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(gy, tmp + nn, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(gz, tmp + 2 * nn, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(tmp);
-  if (e != hipSuccess) return fail(std::string("fxg_get_gradients: ") + hipGetErrorString(e));
+  const size_t nodes = (size_t)c->np * c->K;
+  if (!c->gradOut && alloc_owned(c, &c->gradOut, 5 * nodes)) return 1;
+  if (nVarLifting == 5) {
+    if (!c->grads_state)
+      return fail("fxg_get_gradients: no right-hand side has been evaluated for the current state");
+    if (!c->rhoG && alloc_owned(c, &c->rhoG, 3 * nodes)) return 1;
+    HIPCHECK(fxg::launch_lift_density(c->m, c->w, c->gas, c->p.nodeType, c->grads_state, c->rhoG, c->stream));
+  }
+  double* host[3] = {gx, gy, gz};
+  for (int d = 0; d < 3; ++d) {
+    HIPCHECK(fxg::launch_unpack_gradient_dir(c->m, c->w.G, c->p.precision == FXG_PREC_FP32_BULK, c->rhoG, nVarLifting,
+                                             d, c->gradOut, c->stream));
+    HIPCHECK(hipMemcpyAsync(host[d], c->gradOut, nVarLifting * nodes * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    // the next direction reuses gradOut
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
   return 0;
 }
 
-int fxg_tgv_analyze(fxg_ctx* c, int NAnalyze, double* out) {
-  if (!c || !out) return fail("NULL argument");
+}  // extern "C"
+
+namespace {
+
+// AnalyzeTestcase on the device with the given scratch: part(TGV_NSUMS, K) partial sums,
+// tab(TGV_TAB_MAX) the interpolation table, sums(TGV_NSUMS) the results. Allocates nothing itself
+// (the refresh of a step's gradients uses the context's Uprev, allocated at creation).
+int tgv_analyze_impl(fxg_ctx* c, int NAnalyze, double* out, double* part, double* tabDev, double* sumsDev) {
   if (NAnalyze < 1 || NAnalyze > fxg::TGV_NANALYZE_MAX)
     return fail("fxg_tgv_analyze: NAnalyze must be in 1.." + std::to_string(fxg::TGV_NANALYZE_MAX));
   if (refresh_last_rhs(c)) return 1;
-  const int n = c->n, na = NAnalyze + 1, nt = (fxg::TGV_NANALYZE_MAX + 1) * (n + 1) + n;
-  if (!c->anaPart && (alloc_owned(c, &c->anaPart, (size_t)fxg::TGV_NSUMS * c->K) ||
-                      alloc_owned(c, &c->anaTab, (size_t)nt) || alloc_owned(c, &c->anaOut, (size_t)fxg::TGV_NSUMS)))
-    return 1;
+  const int n = c->n, na = NAnalyze + 1;
   // V(na, n): Lagrange polynomials of the solution nodes at the NAnalyze Gauss-Lobatto points
   // (Vdm_GaussN_NAnalyze, analyze.f90:258-264), then their weights and the solution weights
-  std::vector<double> x(n), w(n), wb(n), xa(na), wa(na), L(n), tab((size_t)na * n + na + n);
-  fxg::basis::nodes(c->N, c->p.nodeType == FXG_NODE_GAUSS ? fxg::basis::GAUSS : fxg::basis::GAUSS_LOBATTO, x.data(),
-                    w.data());
-  fxg::basis::bary_weights(c->N, x.data(), wb.data());
-  fxg::basis::nodes(NAnalyze, fxg::basis::GAUSS_LOBATTO, xa.data(), wa.data());
+  double x[8], w[8], wb[8], xa[fxg::TGV_NANALYZE_MAX + 1], wa[fxg::TGV_NANALYZE_MAX + 1], L[8];
+  double tab[fxg::TGV_TAB_MAX];
+  fxg::basis::nodes(c->N, c->p.nodeType == FXG_NODE_GAUSS ? fxg::basis::GAUSS : fxg::basis::GAUSS_LOBATTO, x, w);
+  fxg::basis::bary_weights(c->N, x, wb);
+  fxg::basis::nodes(NAnalyze, fxg::basis::GAUSS_LOBATTO, xa, wa);
   for (int a = 0; a < na; ++a) {
-    fxg::basis::lagrange_at(xa[a], c->N, x.data(), wb.data(), L.data());
+    fxg::basis::lagrange_at(xa[a], c->N, x, wb, L);
     for (int i = 0; i < n; ++i) tab[(size_t)a * n + i] = L[i];
     tab[(size_t)na * n + a] = wa[a];
   }
   for (int i = 0; i < n; ++i) tab[(size_t)na * n + na + i] = w[i];
+  const size_t ntab = (size_t)na * n + na + n;
   double sums[fxg::TGV_NSUMS];
-  HIPCHECK(hipMemcpyAsync(c->anaTab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(fxg::launch_tgv_analyze(c->m, c->U, c->w.G, c->p.precision == FXG_PREC_FP32_BULK, c->anaTab, na, c->gas,
-                                   c->anaPart, c->anaOut, c->stream));
-  HIPCHECK(hipMemcpyAsync(sums, c->anaOut, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(tabDev, tab, ntab * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(fxg::launch_tgv_analyze(c->m, c->U, c->w.G, c->p.precision == FXG_PREC_FP32_BULK, tabDev, na, c->gas,
+                                   part, sumsDev, c->stream));
+  HIPCHECK(hipMemcpyAsync(sums, sumsDev, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   // normalisation (testcase.f90:443-459), rho0 = 1 (testcase_vars.f90:35)
   const double rho0 = 1.0, Vol = sums[12], mu0 = c->gas.mu0;
@@ -614,6 +645,19 @@ int fxg_tgv_analyze(fxg_ctx* c, int NAnalyze, double* out) {
                         sums[9] / Vol, sums[10] * 4.0 / 3.0 / Vol};
   for (int q = 0; q < 15; ++q) out[q] = v[q];
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fxg_tgv_analyze(fxg_ctx* c, int NAnalyze, double* out) {
+  if (!c || !out) return fail("NULL argument");
+  if (!c->anaPart && (alloc_owned(c, &c->anaPart, (size_t)fxg::TGV_NSUMS * c->K) ||
+                      alloc_owned(c, &c->anaTab, (size_t)fxg::TGV_TAB_MAX) ||
+                      alloc_owned(c, &c->anaOut, (size_t)fxg::TGV_NSUMS)))
+    return 1;
+  return tgv_analyze_impl(c, NAnalyze, out, c->anaPart, c->anaTab, c->anaOut);
 }
 
 int fxg_get_metrics(fxg_ctx* c, double* Mf, double* Mg, double* Mh, double* sJ, double* NV, double* T1, double* T2,
@@ -634,7 +678,10 @@ int fxg_dg_time_derivative(fxg_ctx* c, double t) {
   if (!c) return fail("ctx is NULL");
   fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
   c->rhs_stale = false;
+  c->last_rhs = 1;
+  c->grads_state = nullptr;
   if (rhs(c, c->U, c->Ut, st, c->stream)) return 1;
+  c->grads_state = c->U;
   HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -645,8 +692,11 @@ int fxg_timestep_lserkw2(fxg_ctx* c, double t, double dt, int nStages, const dou
   (void)RKc;  // stage times only enter time-dependent sources or boundary states, which this path has none of
   if (!c) return fail("ctx is NULL");
   c->lam_of = nullptr;
+  c->grads_state = nullptr;
   if (lserk(c, dt, nStages, RKA, RKb, c->U, c->R, c->stream, c->fused && c->faces_of == c->U, true)) return 1;
-  c->rhs_stale = c->fused;
+  // Ut of the last stage is not kept by either pipeline, the gradients not by the fused one
+  c->rhs_stale = true;
+  c->last_rhs = 2;
   if (c->fused) c->lam_of = c->U;
   c->last_b_dt = RKb[nStages - 1] * dt;
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -659,6 +709,9 @@ int fxg_dg_time_derivative_dev(fxg_ctx* c, double t, const double* U, double* Ut
   fxg::RKStage st{0, 0.0, 0.0, nullptr, nullptr};
   const int rc = rhs(c, U, Ut, st, stream ? (hipStream_t)stream : c->stream);
   c->faces_of = nullptr;  // the caller's buffer may change before the next call
+  // G now holds the gradients of the caller's state: the context's are recomputed on request
+  c->rhs_stale = true;
+  c->grads_state = nullptr;
   return rc;
 }
 
@@ -669,6 +722,10 @@ int fxg_timestep_lserkw2_dev(fxg_ctx* c, double t, double dt, int nStages, const
   if (!c || !U || !R) return fail("NULL argument");
   const int rc = lserk(c, dt, nStages, RKA, RKb, U, R, stream ? (hipStream_t)stream : c->stream);
   c->faces_of = nullptr;
+  if (!c->fused) {  // the four-kernel stages wrote the caller's gradients into G
+    c->rhs_stale = true;
+    c->grads_state = nullptr;
+  }
   return rc;
 }
 

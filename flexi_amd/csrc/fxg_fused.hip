@@ -46,7 +46,10 @@ struct LdsB {
 // code, which keeps the kernel within 128 VGPRs (two workgroups per CU at N=7) without spills.
 // In FP32-bulk the LDS arrays are FP32 (38 KB at N=7) and the kernel is held to 64 VGPRs, so
 // four workgroups share a CU (six waves per SIMD below N=7, where 8 would spill).
-template <int N, class RF, int SOLVER, bool BC>
+// LAM: the last stage of a step, which also forms CALCTIMESTEP's per-element spectral radii of the
+// new state; in FP64 from the metric terms the direction loop loaded (the same values as the
+// time-step kernel reads), so they are not read from memory a second time.
+template <int N, class RF, int SOLVER, bool BC, bool LAM>
 __global__ __launch_bounds__(D<N>::NT)
 __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
                                                      double* __restrict__ FUnext, const RF* __restrict__ FG,
@@ -213,12 +216,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
   // the register R and the state U of the update are loaded before the last direction (whose
   // arithmetic then hides their latency; fv of the first two directions no longer holds registers)
   double rpre[5] = {0, 0, 0, 0, 0}, upre[5] = {0, 0, 0, 0, 0};
+  constexpr bool keepJ = LAM && sizeof(RF) == 8;
+  double Jk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // Ja^1..3 of the node for the spectral radii (keepJ)
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     RF m0 = 0, m1 = 0, m2 = 0;
     if (act) {
       const double* Jg = (d == 0 ? m.Mf : (d == 1 ? m.Mg : m.Mh)) + 3 * gid;
       m0 = (RF)Jg[0]; m1 = (RF)Jg[1]; m2 = (RF)Jg[2];
+      if constexpr (keepJ) {
+        Jk[3 * d] = (double)m0; Jk[3 * d + 1] = (double)m1; Jk[3 * d + 2] = (double)m2;
+      }
       sM[t] = m0; sM[np + t] = m1; sM[2 * np + t] = m2;
 #pragma unroll
       for (int c = 0; c < 4; ++c) sF[c * np + t] = fv[4 * d + c];
@@ -293,11 +301,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
       }
     }
   }
-  if (st.lamSum) {  // uniform over the launch; the direction loop's last barrier freed sX
+  if constexpr (LAM) {  // the direction loop's last barrier freed sX
     double lam[6] = {0, 0, 0, 0, 0, 0};
     int isnan_rho = 0;
-    if (act)
-      dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
+    if (act) {
+      if constexpr (keepJ) dt_node_lambda(un, gas, sjd, Jk, Jk + 3, Jk + 6, lam, isnan_rho);
+      else dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
+    }
     double mx[6];
     int nanr = 0;
     double* sRd = reinterpret_cast<double*>(sX);
@@ -367,9 +377,13 @@ hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int rie
     const double* FUc = w.FU[cur];
     double* FUn = w.FU[1 - cur];
     const RF* FG = (const RF*)w.FG;
-#define FXG_LAUNCH(SOLV, BCV)                                                                                   \
-  hipLaunchKernelGGL((k_dg_fused<N, RF, SOLV, BCV>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, st, \
-                     (RF*)G, w.bcP)
+#define FXG_LAUNCH(SOLV, BCV)                                                                                       \
+  if (st.lamSum)                                                                                                   \
+    hipLaunchKernelGGL((k_dg_fused<N, RF, SOLV, BCV, true>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut,  \
+                       st, (RF*)G, w.bcP);                                                                         \
+  else                                                                                                             \
+    hipLaunchKernelGGL((k_dg_fused<N, RF, SOLV, BCV, false>), dim3(K), dim3(NT), 0, s, U, m, FUc, FUn, FG, gas, Ut, \
+                       st, (RF*)G, w.bcP)
     if (riemann == 1) {
       if (bc) FXG_LAUNCH(1, true); else FXG_LAUNCH(1, false);
     } else if (riemann == 3) {

@@ -288,8 +288,12 @@ int load_mesh(const char* path, MeshData& md, bool withNodes, bool allowBC = fal
     const size_t per = (size_t)3 * (NGeo + 1) * (NGeo + 1) * (NGeo + 1);
     md.nodes.resize(per * K);
     for (int e = 0; e < K; ++e) {
-      const size_t first = (size_t)ei[(size_t)6 * e + 4];
-      if (3 * first + per > all.size()) return fail("NodeCoords index out of range");
+      // FirstNodeInd is an offset read from the file: reject negative values, and compare
+      // without forming 3*first + per, which could wrap around
+      const long long f = ei[(size_t)6 * e + 4];
+      if (f < 0 || (unsigned long long)f > all.size() / 3 || per > all.size() - 3 * (size_t)f)
+        return fail("NodeCoords index out of range");
+      const size_t first = (size_t)f;
       std::memcpy(&md.nodes[per * e], &all[3 * first], per * sizeof(double));
     }
   }

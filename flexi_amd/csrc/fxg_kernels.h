@@ -171,6 +171,14 @@ hipError_t launch_calc_timestep_from_sums(const DevMesh& m, const DevWork& w, co
 // FP64 arrays (4,0:N,0:N,0:N,nElems).
 hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, double* gx, double* gy,
                                    double* gz, hipStream_t s);
+// Lifted density gradient rhoG(np, 3, nElems) of U (PP_OPTLIFT=0's LIFT_DENS), through the face
+// states written into w.Uf_m/Uf_s (overwritten).
+hipError_t launch_lift_density(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, const double* U,
+                               double* rhoG, hipStream_t s);
+// Direction `dir` of the lifted gradients in FLEXI's layout (nVar,0:N,0:N,0:N,nElems): nVar 4
+// (u,v,w,T) from G, or 5 with rho first from rhoG.
+hipError_t launch_unpack_gradient_dir(const DevMesh& m, const void* G, int fp32, const double* rhoG, int nVar, int dir,
+                                      double* out, hipStream_t s);
 
 // Taylor-Green vortex analysis (AnalyzeTestcase): per-element partial sums part(13, nElems) of
 // the integrals at na^3 Gauss-Lobatto points, then their sum (maximum for the vorticity) over
@@ -179,6 +187,9 @@ hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, do
 // (Lagrange polynomials of the solution nodes at the analysis points), wA(na), wGP(n).
 constexpr int TGV_NANALYZE_MAX = 31;
 constexpr int TGV_NSUMS = 13;
+// interpolation table of the analysis: V(na, n), wA(na), wGP(n), at most this many doubles (N <= 7)
+constexpr int TGV_TAB_MAX = (TGV_NANALYZE_MAX + 1) * (8 + 1) + 8;
+
 hipError_t launch_tgv_analyze(const DevMesh& m, const double* U, const void* G, int fp32, const double* tab, int na,
                               const Gas& gas, double* part, double* out, hipStream_t s);
 bool supported_degree(int N);

@@ -646,6 +646,88 @@ __global__ __launch_bounds__(256) void k_unpack_grad(const RF* __restrict__ G, i
 }
 
 // ---------------------------------------------------------------------------------------------
+// Lifted density gradient, the first lifting variable of FLEXI's default build (PP_OPTLIFT=0,
+// LIFT_DENS, idealgas/eos.h:108-117): the BR1 lifting of k_lift applied to rho (strong form,
+// lifting_br1.t90:48-181) with the boundary values of Lifting_GetBoundaryFlux
+// (getboundaryflux.f90:855-939): Dirichlet 0.5*(rho_in + rho_b), no-slip walls the wall state's
+// density, slip walls the inner density. The solver itself does not need it (its viscous fluxes
+// use only the velocity and temperature gradients); FP64 in both precisions.
+// rhoG(np, 3, nElems): d/dx, d/dy, d/dz of rho at each node.
+template <int N, bool GL>
+__global__ __launch_bounds__(Dim<N>::NT) void k_lift_density(const double* __restrict__ U, DevMesh m,
+                                                           const double* __restrict__ Um,
+                                                           const double* __restrict__ Us, Gas gas,
+                                                           double* __restrict__ rhoG) {
+  constexpr int n = Dim<N>::n, nf = Dim<N>::nf, np = Dim<N>::np, NT = Dim<N>::NT;
+  __shared__ double sR[np];
+  __shared__ double sB[3 * n * n + 4 * n];
+  const int e = blockIdx.x;
+  const int t = threadIdx.x;
+  if (t < np) sR[t] = U[(size_t)5 * ((size_t)np * e + t)];
+  load_basis<N>(m.basis, sB, t, NT);
+  __syncthreads();
+  if (t >= np) return;
+  const BasisLayout B{n};
+  const int i = t % n, j = (t / n) % n, k = t / (n * n);
+  const double* DT = sB + B.D_T();
+  double gxi = DT[0 + n * i] * sR[0 + n * (j + n * k)];
+  double geta = DT[0 + n * j] * sR[i + n * (0 + n * k)];
+  double gzeta = DT[0 + n * k] * sR[i + n * (j + n * 0)];
+  for (int l = 1; l <= N; ++l) {
+    gxi = gxi + DT[l + n * i] * sR[l + n * (j + n * k)];
+    geta = geta + DT[l + n * j] * sR[i + n * (l + n * k)];
+    gzeta = gzeta + DT[l + n * k] * sR[i + n * (j + n * l)];
+  }
+  const size_t gid = (size_t)np * e + t;
+  double g[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) g[d] = m.Mf[3 * gid + d] * gxi + m.Mg[3 * gid + d] * geta + m.Mh[3 * gid + d] * gzeta;
+#pragma unroll 1
+  for (int ls = 1; ls <= 6; ++ls) {
+    int a, b, l;
+    face_coords(ls, i, j, k, a, b, l);
+    if (GL && l != (is_minus(ls) ? 0 : N)) continue;
+    const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
+    const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
+    const int* pq2 = m.v2s2 + 2 * ((a + n * b) + nf * (flip + 5 * (ls - 1)));
+    const int fp = pq2[0] + n * pq2[1] + nf * side;
+    const double se = m.SurfElem[fp];
+    const double lh = GL ? (is_minus(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N])
+                         : (is_minus(ls) ? sB[B.L_HatMinus() + l] : sB[B.L_HatPlus() + l]);
+    double f;
+    if (side < m.nBC) {
+      double Pm[6], Pb[6], nv[3], t1[3], t2[3];
+      const int type = bc_point(m, Um, gas, fp, Pm, Pb, nv, t1, t2);
+      const double rb = type == BC_DIRICHLET ? 0.5 * (Pm[0] + Pb[0]) : (type == BC_WALL_SLIP ? Pm[0] : Pb[0]);
+      f = (rb - Pm[0]) * se;
+    } else {
+      f = 0.5 * (Us[5 * (size_t)fp] - Um[5 * (size_t)fp]) * se;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[d] = g[d] + f * m.NormVec[3 * (size_t)fp + d] * lh;
+  }
+  const double sj = m.sJ[gid];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) rhoG[t + (size_t)np * (d + 3 * (size_t)e)] = g[d] * sj;
+}
+
+// One direction of the lifted gradients in FLEXI's layout gradU{x,y,z}(nVar,0:N,0:N,0:N,nElems):
+// nVar = 4 (u,v,w,T; PP_OPTLIFT=1) or 5 (rho first, from rhoG; PP_OPTLIFT=0).
+template <class RF>
+__global__ __launch_bounds__(256) void k_unpack_grad_dir(const RF* __restrict__ G, const double* __restrict__ rhoG,
+                                                        int np, int nNodes, int nVar, int dir,
+                                                        double* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nNodes) return;
+  const int e = x / np, t = x - e * np;
+  const RF* Ge = G + (size_t)12 * np * e;
+  double* o = out + (size_t)nVar * x;
+  if (nVar == 5) *o++ = rhoG[t + (size_t)np * (dir + 3 * (size_t)e)];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) o[v] = (double)Ge[(4 * dir + v) * np + t];
+}
+
+// ---------------------------------------------------------------------------------------------
 // Dispatch on the compile-time degree.
 
 struct TimedScope {
@@ -782,6 +864,42 @@ hipError_t launch_calc_timestep(const DevMesh& m, const DevWork& w, const Gas& g
   }
 }
 
+
+hipError_t launch_lift_density(const DevMesh& m, const DevWork& w, const Gas& gas, int nodeType, const double* U,
+                               double* rhoG, hipStream_t s) {
+  const bool gl = nodeType == 2;
+  switch (m.N) {
+#define FXG_CASE(NN)                                                                                               \
+  case NN:                                                                                                         \
+    if (gl) {                                                                                                      \
+      hipLaunchKernelGGL((k_prolong_u<NN, true>), dim3(m.nElems), dim3(Dim<NN>::NT), 0, s, U, m, w.Uf_m, w.Uf_s);   \
+      hipLaunchKernelGGL((k_lift_density<NN, true>), dim3(m.nElems), dim3(Dim<NN>::NT), 0, s, U, m, w.Uf_m, w.Uf_s, \
+                         gas, rhoG);                                                                               \
+    } else {                                                                                                       \
+      hipLaunchKernelGGL((k_prolong_u<NN, false>), dim3(m.nElems), dim3(Dim<NN>::NT), 0, s, U, m, w.Uf_m, w.Uf_s);  \
+      hipLaunchKernelGGL((k_lift_density<NN, false>), dim3(m.nElems), dim3(Dim<NN>::NT), 0, s, U, m, w.Uf_m,        \
+                         w.Uf_s, gas, rhoG);                                                                       \
+    }                                                                                                              \
+    break;
+    FXG_CASE(1) FXG_CASE(2) FXG_CASE(3) FXG_CASE(4) FXG_CASE(5) FXG_CASE(6) FXG_CASE(7)
+#undef FXG_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_gradient_dir(const DevMesh& m, const void* G, int fp32, const double* rhoG, int nVar, int dir,
+                                      double* out, hipStream_t s) {
+  const int n = m.N + 1, np = n * n * n;
+  const int nNodes = np * m.nElems;
+  if (fp32)
+    hipLaunchKernelGGL(k_unpack_grad_dir<float>, dim3((nNodes + 255) / 256), dim3(256), 0, s, (const float*)G, rhoG,
+                       np, nNodes, nVar, dir, out);
+  else
+    hipLaunchKernelGGL(k_unpack_grad_dir<double>, dim3((nNodes + 255) / 256), dim3(256), 0, s, (const double*)G, rhoG,
+                       np, nNodes, nVar, dir, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_unpack_gradients(const DevMesh& m, const void* G, int fp32, double* gx, double* gy, double* gz,
                                    hipStream_t s) {

@@ -32,12 +32,12 @@ TYPE,BIND(C),PUBLIC :: fxg_mesh
   TYPE(C_PTR)    :: NormVec, TangVec1, TangVec2, SurfElem
 END TYPE fxg_mesh
 
-PUBLIC :: fxg_last_error, fxg_version, fxg_create, fxg_create_from_nodes, fxg_destroy
+PUBLIC :: fxg_last_error, fxg_version, fxg_create, fxg_create_from_nodes, fxg_destroy, fxg_set_boundaries
 PUBLIC :: fxg_set_state, fxg_get_state, fxg_get_ut, fxg_get_gradients, fxg_get_metrics
 PUBLIC :: fxg_tgv_analyze
 PUBLIC :: fxg_dg_time_derivative, fxg_timestep_lserkw2, fxg_calc_timestep
 PUBLIC :: fxg_dg_time_derivative_dev, fxg_timestep_lserkw2_dev, fxg_pipeline, fxg_sync, fxg_sizes
-PUBLIC :: fxg_rhs_vjp_dev, fxg_forward_n_steps, fxg_adjoint_n_steps
+PUBLIC :: fxg_rhs_vjp_dev, fxg_forward_n_steps, fxg_adjoint_n_steps, fxg_adjoint_storage
 PUBLIC :: fxg_profile_enable, fxg_profile_read, fxg_string
 
 INTERFACE
@@ -91,9 +91,11 @@ INTERFACE
     TYPE(C_PTR),VALUE          :: ctx
     REAL(C_DOUBLE),INTENT(OUT) :: Ut(*)
   END FUNCTION
-  INTEGER(C_INT) FUNCTION fxg_get_gradients(ctx,gx,gy,gz) BIND(C,NAME='fxg_get_gradients')
+  ! gradUx/y/z(nVarLifting,0:N,0:N,0:N,nElems): nVarLifting = PP_nVarLifting of the host build
+  INTEGER(C_INT) FUNCTION fxg_get_gradients(ctx,nVarLifting,gx,gy,gz) BIND(C,NAME='fxg_get_gradients')
     IMPORT :: C_INT, C_DOUBLE, C_PTR
     TYPE(C_PTR),VALUE          :: ctx
+    INTEGER(C_INT),VALUE       :: nVarLifting
     REAL(C_DOUBLE),INTENT(OUT) :: gx(*), gy(*), gz(*)
   END FUNCTION
   ! the 15 values of writeBuf in AnalyzeTestcase (testcase.f90:464-466), evaluated on the device

@@ -115,10 +115,13 @@ class DGSolver:
         _lib.check(_lib.lib().fxg_get_ut(self.ctx, _ptr(Ut)))
         return Ut
 
-    def gradients(self):
+    def gradients(self, n_var_lifting: int = 4):
+        """gradUx, gradUy, gradUz (n_var_lifting,0:N,0:N,0:N,nElems) of the last right-hand side:
+        4 lifting variables (u,v,w,T) as FLEXI's PP_OPTLIFT=1 build holds them, or 5 (rho,u,v,w,T)
+        as its default PP_OPTLIFT=0 build does."""
         n = self.N + 1
-        g = [np.zeros((4, n, n, n, self.mesh.nElems), order="F") for _ in range(3)]
-        _lib.check(_lib.lib().fxg_get_gradients(self.ctx, *[_ptr(x) for x in g]))
+        g = [np.zeros((n_var_lifting, n, n, n, self.mesh.nElems), order="F") for _ in range(3)]
+        _lib.check(_lib.lib().fxg_get_gradients(self.ctx, int(n_var_lifting), *[_ptr(x) for x in g]))
         return g
 
     def tgv_analyze(self, n_analyze: int) -> np.ndarray:

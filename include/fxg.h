@@ -112,12 +112,15 @@ int fxg_set_boundaries(fxg_ctx* ctx, int nBCs, const int* BC, const int* Boundar
 int fxg_set_state(fxg_ctx* ctx, const double* U);
 int fxg_get_state(fxg_ctx* ctx, double* U);
 int fxg_get_ut(fxg_ctx* ctx, double* Ut);
-/* Lifted gradients of the last right-hand side, gradU{x,y,z}(4,0:N,0:N,0:N,nElems) for the
- * lifting variables (u,v,w,T) (src/dg/lifting/lifting_vars.f90; PP_OPTLIFT=1 set, eos.h:114):
- * after fxg_dg_time_derivative those of U; after fxg_timestep_lserkw2, as in FLEXI, those of the
- * last stage's right-hand side (at the state before the last update; the fused Gauss-Lobatto
- * path recomputes them there on request, equal to round-off). fxg_get_ut likewise. */
-int fxg_get_gradients(fxg_ctx* ctx, double* gx, double* gy, double* gz);
+/* Lifted gradients of the last right-hand side, gradU{x,y,z}(nVarLifting,0:N,0:N,0:N,nElems)
+ * (src/dg/lifting/lifting_vars.f90) in the layout of FLEXI's build: nVarLifting = 4 for
+ * PP_OPTLIFT=1, lifting variables (u,v,w,T); 5 for FLEXI's default PP_OPTLIFT=0, (rho,u,v,w,T)
+ * (src/equations/navierstokes/idealgas/eos.h:108-127). After fxg_dg_time_derivative those of U;
+ * after fxg_timestep_lserkw2, as in FLEXI, those of the last stage's right-hand side (at the state
+ * before the last update, recomputed there on request, equal to round-off). The density
+ * gradient, which the solver itself does not need, is lifted on request from the same state.
+ * fxg_get_ut likewise. */
+int fxg_get_gradients(fxg_ctx* ctx, int nVarLifting, double* gx, double* gy, double* gz);
 /* Taylor-Green vortex analysis, AnalyzeTestcase (src/testcase/taylorgreenvortex/testcase.f90:241-479)
  * evaluated on the device: the state and the lifted velocity gradients that fxg_get_gradients
  * would return are interpolated to NAnalyze+1 Gauss-Lobatto points per direction
@@ -126,6 +129,7 @@ int fxg_get_gradients(fxg_ctx* ctx, double* gx, double* gy, double* gz);
  * vorticity, mean temperature, uprime, mean entropy, ED_S, ED_D), constant viscosity, rho0 = 1.
  * Replaces the body of AnalyzeTestcase on a GPU host (no download of U and the gradients). */
 int fxg_tgv_analyze(fxg_ctx* ctx, int NAnalyze, double* out);
+
 /* Geometry as used on the device (for checking fxg_create_from_nodes): same layouts as fxg_mesh
  * plus Elem_xGP(3,0:N,0:N,0:N,nElems). Any pointer may be NULL. */
 int fxg_get_metrics(fxg_ctx* ctx, double* Mf, double* Mg, double* Mh, double* sJ,
@@ -147,7 +151,11 @@ int fxg_calc_timestep(fxg_ctx* ctx, double CFLScale, double DFLScale, double* dt
 
 /* Device-pointer forms for hosts that keep the state in their own device buffers (e.g. torch
  * tensors): the arrays have the FLEXI layouts above and live on the context's device; `stream`
- * is a hipStream_t (NULL = the context's stream). The call is asynchronous on that stream. */
+ * is a hipStream_t (NULL = the context's stream). The call is asynchronous on that stream. It
+ * uses the context's scratch arrays (face states, face fluxes, gradients), so calls on one
+ * context must be serialised: with a stream other than the context's, the caller orders the
+ * calls itself (e.g. synchronises before switching streams). Afterwards fxg_get_ut and
+ * fxg_get_gradients recompute those of the context's own state. */
 int fxg_dg_time_derivative_dev(fxg_ctx* ctx, double t, const double* U_dev, double* Ut_dev,
                                void* stream);
 int fxg_timestep_lserkw2_dev(fxg_ctx* ctx, double t, double dt, int nStages, const double* RKA,

@@ -869,6 +869,72 @@ void or_get_gradients(or_ctx* c, double* gx, double* gy, double* gz) {
   std::memcpy(gz, c->gz.data(), n * sizeof(double));
 }
 
+// The lifted density gradient of FLEXI's default build (PP_OPTLIFT=0, LIFT_DENS first of the
+// PRIM_LIFT variables, idealgas/eos.h:108-117) for the state of the last right-hand side: the
+// steps of Lifting_BR1 above (lifting_br1.t90:48-181; volume lifting_volint.t90:26-118, fluxes
+// lifting_fillflux.t90:40-239) for the one variable rho, with Lifting_GetBoundaryFlux's density
+// (getboundaryflux.f90:890-921: Riemann-type 0.5*(rho_in + rho_b), no-slip walls the boundary
+// state's rho, slip walls the inner rho). gx/gy/gz: (0:N,0:N,0:N,nElems).
+void or_get_density_gradient(or_ctx* c, double* gx, double* gy, double* gz) {
+  const int N = c->N, n = c->n, nf = c->nf, np = c->np, K = c->nElems;
+  const size_t nn = (size_t)np * K;
+  std::fill(gx, gx + nn, 0.0);
+  std::fill(gy, gy + nn, 0.0);
+  std::fill(gz, gz + nn, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < K; ++e) {
+    const double* P = &c->UPrim[(size_t)6 * np * e];
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double gxi = c->D_T[0 + n * i] * P[DENS + 6 * vidx(c, 0, j, k)];
+          double geta = c->D_T[0 + n * j] * P[DENS + 6 * vidx(c, i, 0, k)];
+          double gzeta = c->D_T[0 + n * k] * P[DENS + 6 * vidx(c, i, j, 0)];
+          for (int l = 1; l <= N; ++l) {
+            gxi = gxi + c->D_T[l + n * i] * P[DENS + 6 * vidx(c, l, j, k)];
+            geta = geta + c->D_T[l + n * j] * P[DENS + 6 * vidx(c, i, l, k)];
+            gzeta = gzeta + c->D_T[l + n * k] * P[DENS + 6 * vidx(c, i, j, l)];
+          }
+          const size_t id = (size_t)np * e + vidx(c, i, j, k);
+          const double* mf = &c->Mf[3 * id];
+          const double* mg = &c->Mg[3 * id];
+          const double* mh = &c->Mh[3 * id];
+          gx[id] = mf[0] * gxi + mg[0] * geta + mh[0] * gzeta;
+          gy[id] = mf[1] * gxi + mg[1] * geta + mh[1] * gzeta;
+          gz[id] = mf[2] * gxi + mg[2] * geta + mh[2] * gzeta;
+        }
+  }
+  std::vector<double> fx((size_t)nf * c->nSides), fy(fx.size()), fz(fx.size());
+  for (int s = 0; s < c->nSides; ++s)
+    for (int pq = 0; pq < nf; ++pq) {
+      const size_t fi = (size_t)nf * s + pq;
+      double f;
+      if (s < c->nBCSides) {
+        const int type = bc_type(c, s);
+        double Pb[6];
+        const double* Pm = &c->UPrim_master[6 * fi];
+        boundary_state(c->e, type, bc_ref(c, s), Pm, &c->NormVec[3 * fi], &c->TangVec1[3 * fi], &c->TangVec2[3 * fi],
+                       Pb);
+        const double rb = type == 2 ? 0.5 * (Pm[DENS] + Pb[DENS]) : (type == 9 ? Pm[DENS] : Pb[DENS]);
+        f = (rb - Pm[DENS]) * c->SurfElem[fi];
+      } else {
+        f = -1 * c->UPrim_master[DENS + 6 * fi] + c->UPrim_slave[DENS + 6 * fi];
+        f = 0.5 * f * c->SurfElem[fi];
+      }
+      fx[fi] = f * c->NormVec[0 + 3 * fi];
+      fy[fi] = f * c->NormVec[1 + 3 * fi];
+      fz[fi] = f * c->NormVec[2 + 3 * fi];
+    }
+  surf_int(c, 1, fx.data(), fx.data(), +1.0, gx);
+  surf_int(c, 1, fy.data(), fy.data(), +1.0, gy);
+  surf_int(c, 1, fz.data(), fz.data(), +1.0, gz);
+  for (size_t g = 0; g < nn; ++g) {
+    gx[g] = gx[g] * c->sJ[g];
+    gy[g] = gy[g] * c->sJ[g];
+    gz[g] = gz[g] * c->sJ[g];
+  }
+}
+
 void or_get_faces(or_ctx* c, double* Um, double* Us, double* Fm) {
   const size_t n = 5 * (size_t)c->nf * c->nSides;
   if (Um) std::memcpy(Um, c->U_master.data(), n * sizeof(double));

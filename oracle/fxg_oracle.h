@@ -85,6 +85,8 @@ void or_set_threads(int n);
 void or_dg_time_derivative(or_ctx* c, double t, const double* U, double* Ut);
 /* Lifted gradients of the last RHS call: (4,0:N,0:N,0:N,nElems) for (u,v,w,T) in x,y,z. */
 void or_get_gradients(or_ctx* c, double* gx, double* gy, double* gz);
+// density gradient of the last right-hand side (PP_OPTLIFT=0), (0:N,0:N,0:N,nElems) each
+void or_get_density_gradient(or_ctx* c, double* gx, double* gy, double* gz);
 /* Face arrays of the last RHS call (FLEXI layout). */
 void or_get_faces(or_ctx* c, double* U_master, double* U_slave, double* Flux_master);
 /* CALCTIMESTEP (calctimestep.f90:89-227) with CFLScale/DFLScale already scaled by FillCFL_DFL. */

@@ -67,6 +67,7 @@ def _setup(L):
     L.or_set_threads.argtypes = [C.c_int]
     L.or_dg_time_derivative.argtypes = [v, C.c_double, _dp, _dp]
     L.or_get_gradients.argtypes = [v, _dp, _dp, _dp]
+    L.or_get_density_gradient.argtypes = [v, _dp, _dp, _dp]
     L.or_get_faces.argtypes = [v, _dp, _dp, _dp]
     L.or_calc_timestep.restype = C.c_double
     L.or_calc_timestep.argtypes = [v, _dp, C.c_double, C.c_double, C.POINTER(C.c_int), _dp]
@@ -182,11 +183,17 @@ class Solver:
         lib().or_dg_time_derivative(self.ctx, t, np.asfortranarray(U), Ut)
         return Ut
 
-    def gradients(self):
+    def gradients(self, n_var_lifting=4):
+        """gradUx/y/z of the last right-hand side: (u,v,w,T), or with n_var_lifting=5
+        (rho,u,v,w,T) as FLEXI's default PP_OPTLIFT=0 build holds them."""
         n = self.N + 1
         g = [F((4, n, n, n, self.mesh.nElems)) for _ in range(3)]
         lib().or_get_gradients(self.ctx, *g)
-        return g
+        if n_var_lifting == 4:
+            return g
+        r = [F((n, n, n, self.mesh.nElems)) for _ in range(3)]
+        lib().or_get_density_gradient(self.ctx, *r)
+        return [np.asfortranarray(np.concatenate([ri[None], gi], axis=0)) for ri, gi in zip(r, g)]
 
     def faces(self):
         n = self.N + 1

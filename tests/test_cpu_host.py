@@ -149,6 +149,11 @@ def test_fortran_binding_compiles_links_and_reports_errors(tmp_path):
     _lib.lib()
     mod = os.path.join(ROOT, "flexi_amd", "fortran", "fxg_interface.f90")
     drv = os.path.join(ROOT, "tests", "fortran", "fxg_abi_check.f90")
+    # the check program calls every exported entry point, so a binding that is missing from the
+    # module or not PUBLIC fails to compile
+    src = open(drv).read()
+    missing = [s for s in _lib.EXPORTS if f"{s}(" not in src]
+    assert not missing, missing
     libdir = os.path.dirname(_lib.LIB_PATH)
     subprocess.run([fc, "-c", mod, "-o", str(tmp_path / "m.o"), f"-J{tmp_path}"], check=True, cwd=tmp_path)
     subprocess.run([fc, drv, str(tmp_path / "m.o"), f"-I{tmp_path}", f"-L{libdir}", "-lfxg",

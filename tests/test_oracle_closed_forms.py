@@ -121,6 +121,81 @@ def test_roe_matches_textbook_dissipation_on_1d_jump():
         assert np.isclose(np.sort(w.real)[0], u[0] - c) and np.isclose(np.sort(w.real)[-1], u[0] + c)
 
 
+def normal_flux_jacobian(U, n):
+    """d(f(U).n)/dU by complex-step differentiation (exact to round-off, no cancellation)."""
+    h = 1e-30
+    A = np.zeros((5, 5))
+    for k in range(5):
+        Uc = U.astype(complex)
+        Uc[k] += 1j * h
+        rho, m = Uc[0], Uc[1:4]
+        v = m / rho
+        p = 0.4 * (Uc[4] - 0.5 * (m @ v))
+        vn = v @ n
+        fn = np.r_[rho * vn, m * vn + p * n, (Uc[4] + p) * vn]
+        A[:, k] = fn.imag / h
+    return A
+
+
+def lf_speed_from_eigenvalues(U, n):
+    """(|v_n|, c) of one side from the eigenvalues v_n - c, v_n (x3), v_n + c of its normal flux
+    Jacobian, computed numerically."""
+    w = np.sort(np.linalg.eigvals(normal_flux_jacobian(U, n)).real)
+    return abs(0.5 * (w[0] + w[-1])), 0.5 * (w[-1] - w[0])
+
+
+def state_from(rho, vel, p):
+    vel = np.asarray(vel, float)
+    return np.asfortranarray(np.r_[rho, rho * vel, p / 0.4 + 0.5 * rho * vel @ vel])
+
+
+def test_lax_friedrichs_dissipation_from_jacobian_eigenvalues():
+    """Riemann_LF (riemann.f90:558-589): F = 1/2 (f_L + f_R).n - 1/2 LambdaMax (U_R - U_L) with
+    LambdaMax = max(|v_n,L|, |v_n,R|) + max(c_L, c_R) (:580). Each side's |v_n| and c are read off
+    the eigenvalues of its normal flux Jacobian (numpy, complex-step Jacobian), independently of
+    the oracle's formulas. The states are chosen so that one side has the larger |v_n| and the
+    other the larger sound speed (and the reverse), where the per-side maximum
+    max(|v_n,L| + c_L, |v_n,R| + c_R) is smaller than LambdaMax, and so that one side dominates
+    both; a per-side maximum, a dropped 1/2 or a missing max() fails. The split form only replaces
+    the central part (SplitDGSurface_pointer, :585-587), so F_split - F_weak must be the same for
+    LF as for Roe (pinned by Toro's Roe matrix above)."""
+    rng = np.random.default_rng(7)
+    prm_w = orc.params(3, orc.NODE_GL, 0, orc.RIEMANN_LF, **PRM)
+    prm_s = orc.params(3, orc.NODE_GL, 1, orc.RIEMANN_LF, **PRM)
+    roe_w = orc.params(3, orc.NODE_GL, 0, orc.RIEMANN_ROE, **PRM)
+    roe_s = orc.params(3, orc.NODE_GL, 1, orc.RIEMANN_ROE, **PRM)
+    cases = []
+    for _ in range(6):
+        n, t1, t2 = orthonormal(rng.normal(size=3), rng)
+        tang = rng.normal(size=3)
+        tang -= (tang @ n) * n
+        # fast and cold | slow and hot (c = sqrt(1.4 p / rho))
+        fast_cold = state_from(2.0, 3.0 * n + tang, 0.5)
+        slow_hot = state_from(0.5, -0.2 * n + 0.5 * tang, 4.0)
+        dom = state_from(0.7, -4.0 * n, 6.0)            # larger |v_n| and larger c than both above
+        cases += [(fast_cold, slow_hot, n, t1, t2), (slow_hot, fast_cold, n, t1, t2), (dom, fast_cold, n, t1, t2),
+                  (slow_hot, dom, n, t1, t2)]
+    per_side_differs = 0
+    for UL, UR, n, t1, t2 in cases:
+        fL, PL = euler_flux(UL)
+        fR, PR = euler_flux(UR)
+        vL, cL = lf_speed_from_eigenvalues(UL, n)
+        vR, cR = lf_speed_from_eigenvalues(UR, n)
+        lam = max(vL, vR) + max(cL, cR)
+        per_side_differs += abs(lam - max(vL + cL, vR + cR)) > 0.1 * lam
+        args = (UL, UR, np.asfortranarray(PL), np.asfortranarray(PR), np.asfortranarray(n), np.asfortranarray(t1),
+                np.asfortranarray(t2))
+        F = {}
+        for key, prm in (("lf_w", prm_w), ("lf_s", prm_s), ("roe_w", roe_w), ("roe_s", roe_s)):
+            F[key] = np.zeros(5, order="F")
+            orc.lib().or_riemann_point(prm, *args, F[key])
+        central = 0.5 * (fL + fR) @ n
+        np.testing.assert_allclose(F["lf_w"] - central, -0.5 * lam * (UR - UL), rtol=1e-12,
+                                   atol=1e-12 * lam * np.abs(UR - UL).max())
+        np.testing.assert_allclose(F["lf_s"] - F["lf_w"], F["roe_s"] - F["roe_w"], rtol=1e-12, atol=1e-12)
+    assert per_side_differs >= 12  # the mixed cases do separate FLEXI's LambdaMax from a per-side maximum
+
+
 @pytest.mark.parametrize("N,node,split", [(3, orc.NODE_G, 0), (5, orc.NODE_GL, 1), (4, orc.NODE_GL, 0)])
 def test_freestream_preserved_on_curved_mesh(N, node, split):
     mesh = periodic_box(3, 2, 2, NGeo=N, deform=sine_deform(0.1))
