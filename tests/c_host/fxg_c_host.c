@@ -2,7 +2,10 @@
  * metrics computed on the GPU, restart from a FLEXI State file, nSteps TimeStepByLSERKW2 steps
  * (CarpenterRK4-5, fixed dt), State file out. Only the C ABI (include/fxg.h, include/fxg_io.h).
  *
- *   fxg_c_host <mesh.h5> <N> <nSteps> <dt> <state_in.h5> <state_out.h5>
+ *   fxg_c_host <mesh.h5> <N> <nSteps> <dt> <state_in.h5> <state_out.h5> [<CFLScale> <DFLScale>]
+ *
+ * With CFLScale and DFLScale, each step's dt comes from CALCTIMESTEP first, as in FLEXI's time
+ * loop (src/timedisc/timedisc.f90:160-212), and <dt> is not used.
  *
  * Scheme: Gauss-Lobatto, split form PI, RoeEntropyFix, BR1, kappa 1.4, R 71.42857, Pr 0.72,
  * mu0 6.25e-4 (FLEXI's tgv/split regression parameters). Prints one line "ok t=<t> rho_sum=<sum>"
@@ -23,13 +26,15 @@ static int fail_io(const char* what) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 7) {
-    fprintf(stderr, "usage: %s mesh.h5 N nSteps dt state_in.h5 state_out.h5\n", argv[0]);
+  if (argc != 7 && argc != 9) {
+    fprintf(stderr, "usage: %s mesh.h5 N nSteps dt state_in.h5 state_out.h5 [CFLScale DFLScale]\n", argv[0]);
     return 2;
   }
   const char* meshFile = argv[1];
   const int N = atoi(argv[2]), nSteps = atoi(argv[3]);
-  const double dt = atof(argv[4]);
+  const int cfl = argc == 9;
+  const double cflScale = cfl ? atof(argv[7]) : 0.0, dflScale = cfl ? atof(argv[8]) : 0.0;
+  double dt = atof(argv[4]);
   int K = 0, S = 0, NGeo = 0;
   if (fxg_io_mesh_size(meshFile, &K, &S, &NGeo)) return fail_io("mesh size");
   const int ng = NGeo + 1, n = N + 1;
@@ -60,8 +65,13 @@ int main(int argc, char** argv) {
   const double RKc[4] = {1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
                          2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
   for (int it = 0; it < nSteps; ++it) {
+    if (cfl) {
+      int errType = 0;
+      if (fxg_calc_timestep(ctx, cflScale, dflScale, &dt, &errType, NULL)) return fail_fxg("CALCTIMESTEP");
+    }
     if (fxg_timestep_lserkw2(ctx, t, dt, 5, RKA, RKb, RKc)) return fail_fxg("time step");
     t += dt;
+    if (cfl) printf("dt=%.17e\n", dt);
   }
   if (fxg_get_state(ctx, U)) return fail_fxg("get state");
   if (fxg_io_write_state(argv[6], "c_host", meshFile, nodeType, N, K, t, U)) return fail_io("write state");
