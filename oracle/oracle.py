@@ -14,6 +14,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+# builds of the same sources (oracle/Makefile): "parity" rounds every operation as written (no FMA
+# contraction), "fast" is the -O3 timing build with contraction (its difference from "parity" is
+# the round-off floor of FMA), "native" the timing build for this host (load_native)
+_PATHS = {"parity": _LIB, "fast": os.path.join(_HERE, "liboracle_fast.so")}
 
 NODE_G, NODE_GL, NODE_CL, NODE_VISU = 1, 2, 3, 4
 RIEMANN_LF, RIEMANN_ROE, RIEMANN_ROEENTROPYFIX = 1, 3, 33
@@ -28,17 +32,28 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB):
+def lib(variant: str = "parity"):
+    if variant not in _libs:
+        path = _PATHS[variant]
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(_LIB)
-        _setup(_lib)
-    return _lib
+        L = C.CDLL(path)
+        _setup(L)
+        _libs[variant] = L
+    return _libs[variant]
+
+
+def load_native(out_dir: str) -> str:
+    """Build the -O3 -march=native timing oracle for this host into out_dir (make native) and
+    register it as variant "native"; returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE, "native", f"OUT={out_dir}"], check=True)
+    _PATHS["native"] = os.path.join(out_dir, "liboracle_native.so")
+    _libs.pop("native", None)
+    lib("native")
+    return _PATHS["native"]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
@@ -152,7 +167,8 @@ def rk_coefs():
 class Solver:
     """The oracle's DG operator on one mesh (FLEXI layout)."""
 
-    def __init__(self, prm: Params, mesh, metrics, ref_state_prim=None):
+    def __init__(self, prm: Params, mesh, metrics, ref_state_prim=None, variant="parity"):
+        self.L = lib(variant)
         self.p = prm
         self.N = prm.N
         self.mesh = mesh
@@ -160,27 +176,27 @@ class Solver:
         f = np.asfortranarray
         self._keep = [f(metrics[k]) for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ",
                                                "NormVec", "TangVec1", "TangVec2", "SurfElem")]
-        self.ctx = lib().or_init(C.byref(prm), mesh.nElems, mesh.nSides, f(mesh.SideToElem),
+        self.ctx = self.L.or_init(C.byref(prm), mesh.nElems, mesh.nSides, f(mesh.SideToElem),
                                  f(mesh.ElemToSide), *self._keep)
         n = self.N + 1
         self.shape = (5, n, n, n, mesh.nElems)
         if getattr(mesh, "nBCSides", 0):
             ref = np.asfortranarray(ref_state_prim if ref_state_prim is not None else np.zeros((6, 1)))
             bt = np.asfortranarray(mesh.BoundaryType, dtype=np.int32)
-            rc = lib().or_set_boundaries(self.ctx, mesh.nBCSides, np.asfortranarray(mesh.BC, dtype=np.int32),
+            rc = self.L.or_set_boundaries(self.ctx, mesh.nBCSides, np.asfortranarray(mesh.BC, dtype=np.int32),
                                          bt.shape[0], bt, ref.shape[1], ref)
             if rc != 0:
                 raise ValueError(f"or_set_boundaries: invalid boundary data (code {rc})")
 
     def __del__(self):
         try:
-            lib().or_free(self.ctx)
+            self.L.or_free(self.ctx)
         except Exception:
             pass
 
     def rhs(self, U, t=0.0):
         Ut = F(self.shape)
-        lib().or_dg_time_derivative(self.ctx, t, np.asfortranarray(U), Ut)
+        self.L.or_dg_time_derivative(self.ctx, t, np.asfortranarray(U), Ut)
         return Ut
 
     def gradients(self, n_var_lifting=4):
@@ -188,35 +204,35 @@ class Solver:
         (rho,u,v,w,T) as FLEXI's default PP_OPTLIFT=0 build holds them."""
         n = self.N + 1
         g = [F((4, n, n, n, self.mesh.nElems)) for _ in range(3)]
-        lib().or_get_gradients(self.ctx, *g)
+        self.L.or_get_gradients(self.ctx, *g)
         if n_var_lifting == 4:
             return g
         r = [F((n, n, n, self.mesh.nElems)) for _ in range(3)]
-        lib().or_get_density_gradient(self.ctx, *r)
+        self.L.or_get_density_gradient(self.ctx, *r)
         return [np.asfortranarray(np.concatenate([ri[None], gi], axis=0)) for ri, gi in zip(r, g)]
 
     def faces(self):
         n = self.N + 1
         f = [F((5, n, n, self.mesh.nSides)) for _ in range(3)]
-        lib().or_get_faces(self.ctx, *f)
+        self.L.or_get_faces(self.ctx, *f)
         return f
 
     def calc_timestep(self, U, CFL, DFL):
         err = C.c_int()
         dtElem = F(self.mesh.nElems)
-        dt = lib().or_calc_timestep(self.ctx, np.asfortranarray(U), CFL, DFL, C.byref(err), dtElem)
+        dt = self.L.or_calc_timestep(self.ctx, np.asfortranarray(U), CFL, DFL, C.byref(err), dtElem)
         return dt, err.value, dtElem
 
     def step(self, U, t, dt, R=None):
         """One LSERK step in place on ``U`` (Fortran-ordered float64)."""
         if R is None:
             R = F(self.shape)
-        lib().or_timestep_lserkw2(self.ctx, t, dt, U, R)
+        self.L.or_timestep_lserkw2(self.ctx, t, dt, U, R)
         return U
 
     def tgv_analyze(self, U, grads, NAnalyze=10):
         out = F(15)
-        lib().or_tgv_analyze(self.ctx, np.asfortranarray(U), *grads, NAnalyze, out)
+        self.L.or_tgv_analyze(self.ctx, np.asfortranarray(U), *grads, NAnalyze, out)
         return out
 
 
@@ -249,5 +265,5 @@ def surf_int_raw(N, nodeType, S2E, L_HatMinus, L_HatPlus, Fm, Fs, nElems, slave_
     return Ut
 
 
-def set_threads(n: int) -> None:
-    lib().or_set_threads(n)
+def set_threads(n: int, variant: str = "parity") -> None:
+    lib(variant).or_set_threads(n)
