@@ -49,6 +49,7 @@ B_STAGE = 432.0
 # stages of a step), from rocprofv3 FETCH_SIZE (x2.0, plain-copy calibration) + WRITE_SIZE runs of
 # this benchmark; the summary and the commands are in profiles/fused_stage_tgv32_n7_fp64.md
 MEASURED_BYTES_PER_DOF = {"dg_fused": 420.0, "visc_faces": 140.6, "faces": 61.3}
+RHS_EVALS = 1000
 
 
 def make_solver(nel, precision):
@@ -74,14 +75,14 @@ def run_gpu(nel, precision, steps, warmup, profile):
     s.sync()
     wall = time.perf_counter() - t0
     kt = s.kernel_times() if profile else None
-    # RHS-only rate (DGTimeDerivative_weakForm alone), 50 evaluations
+    # RHS-only rate (DGTimeDerivative_weakForm alone), 1000 evaluations as the configuration asks
     s.profile(False)
     s.sync()
     r0 = time.perf_counter()
-    for _ in range(50):
+    for _ in range(RHS_EVALS):
         s.DGTimeDerivative_weakForm(t)
     s.sync()
-    rhs_wall = (time.perf_counter() - r0) / 50
+    rhs_wall = (time.perf_counter() - r0) / RHS_EVALS
     # the TGV testcase analysis on the device (NAnalyze = 10 as in the TGV parameter files), after
     # the right-hand sides above, i.e. without a recomputation of the last stage's gradients
     a0 = time.perf_counter()
@@ -138,9 +139,9 @@ def fp32_accuracy():
 def adjoint_ratio():
     """Gradient of the kinetic energy at T w.r.t. U(0): TGV 16^3, N=5 Gauss-Lobatto split PI, Roe,
     200 steps (dt frozen from the primal run). `ratio` = time of the adjoint (backward) call / time
-    of the 200 primal forward steps, with the trajectory the forward stores in HBM (71 GB: every
-    stage state and its face states); `checkpointed` = the same with checkpoints every 20 steps,
-    the backward recomputing the stage states of each segment."""
+    of the 200 primal forward steps, with checkpoints every 20 steps (config 5), the backward
+    recomputing the stage states of each segment; `trajectory` = the same with the whole trajectory
+    stored by the forward in HBM (71 GB: every stage state and its face states), no recomputation."""
     import torch
     from flexi_amd import testcase
     from flexi_amd.adjoint import primal_dts, to_tensor
@@ -186,17 +187,16 @@ def adjoint_ratio():
     state = U0.nbytes
     face_states = 10 * (5 + 1) ** 2 * mesh.nSides * 8  # both slots of every side, 5 doubles per point
     s.close()
-    return {"ratio": t_bwd_tr / t_fwd, "mode": "trajectory stored by the forward (HBM), no recomputation",
-            "forward_s": t_fwd, "forward_with_trajectory_s": t_fwd_tr, "adjoint_s": t_bwd_tr,
-            "gradient_total_ratio": (t_fwd_tr + t_bwd_tr) / t_fwd, "trajectory_bytes": traj_bytes,
-            "gradient_bitwise_equal_to_checkpointed": same,
-            "checkpointed": {"ratio": t_bwd / t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "adjoint_s": t_bwd,
-                             "gradient_total_ratio": (t_fwd_ck + t_bwd) / t_fwd,
-                             "memory_bytes": {"checkpoints": (len(dts) // every) * state,
-                                              "segment_stage_states": every * 5 * state,
-                                              "segment_face_states": every * 5 * face_states, "work": 3 * state}},
-            "case": "TGV 16^3, N=5 GL split PI Roe, 200 steps; checkpointed: checkpoints every 20 steps, "
-                    "segments recomputed"}
+    return {"ratio": t_bwd / t_fwd, "mode": "checkpoints every 20 steps (config 5), segments recomputed",
+            "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "adjoint_s": t_bwd,
+            "gradient_total_ratio": (t_fwd_ck + t_bwd) / t_fwd,
+            "memory_bytes": {"checkpoints": (len(dts) // every) * state, "segment_stage_states": every * 5 * state,
+                             "segment_face_states": every * 5 * face_states, "work": 3 * state},
+            "trajectory": {"ratio": t_bwd_tr / t_fwd, "mode": "trajectory stored by the forward (HBM), no "
+                           "recomputation", "forward_with_trajectory_s": t_fwd_tr, "adjoint_s": t_bwd_tr,
+                           "gradient_total_ratio": (t_fwd_tr + t_bwd_tr) / t_fwd, "trajectory_bytes": traj_bytes,
+                           "gradient_bitwise_equal_to_checkpointed": same},
+            "case": "TGV 16^3, N=5 GL split PI Roe, 200 steps, dt frozen from the primal run"}
 
 
 def boundaries(nel=32, steps=20, warmup=3):
@@ -293,46 +293,88 @@ def curvilinear():
             "unit": "DOF-stage/s", "dof": dof, "ms_per_step": 1e3 * wall / 50}
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(nel):
-    """The CPU oracle (a C++/OpenMP restatement of the reference) on one LSERK step of the same
-    workload, on the host cores of this machine."""
+    """The CPU oracle (a C++/OpenMP restatement of the reference, parallel over elements) built for
+    this host with -O3 -march=native (oracle/Makefile `native`, FMA contraction allowed; the
+    parity build is separate) and timed on its cores: after one warm-up, the median of 5 runs of
+    each case. Config 2 (N=7 GL split PI, Roe, BR1, TGV): the right-hand side alone on the full
+    nel^3 mesh, and whole CarpenterRK4-5 steps (CALCTIMESTEP + 5 stages) on a 16^3 sample of the
+    same scheme; config 1 (8^3, N=3 Gauss, weak form, LF, BR1): the same two figures, steps in runs
+    of 50 as the config. Also on one core (config 2 on an 8^3 sample). DOF-stage/s."""
+    import statistics
+    import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    from flexi_amd.mesh import periodic_box
     from flexi_amd import testcase
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    orc.set_threads(threads)
-    mesh = periodic_box(nel, nel, nel)
-    prm = orc.params(7, orc.NODE_GL, 1, orc.RIEMANN_ROE)
-    m = orc.calc_metrics(7, orc.NODE_GL, mesh)
-    S = orc.Solver(prm, mesh, m)
-    U = testcase.tgv(m["Elem_xGP"])
+    from flexi_amd.mesh import periodic_box
     from flexi_amd.timedisc import fill_cfl_dfl
-    cfl, dfl = fill_cfl_dfl(7, 2, 0.8, 0.8)
-    R = orc.F(U.shape)
-    t0 = time.perf_counter()
-    dt, err, _ = S.calc_timestep(U, cfl, dfl)
-    S.step(U, 0.0, dt, R)
-    wall = time.perf_counter() - t0
-    dof = mesh.nElems * 8 ** 3
-    del S
-    # one core, on an 8^3-element sample of the same scheme (one step)
-    orc.set_threads(1)
-    mesh1 = periodic_box(8, 8, 8)
-    m1 = orc.calc_metrics(7, orc.NODE_GL, mesh1)
-    S1 = orc.Solver(prm, mesh1, m1)
-    U1 = testcase.tgv(m1["Elem_xGP"])
-    R1 = orc.F(U1.shape)
-    t0 = time.perf_counter()
-    dt1, _, _ = S1.calc_timestep(U1, cfl, dfl)
-    S1.step(U1, 0.0, dt1, R1)
-    wall1 = time.perf_counter() - t0
-    orc.set_threads(threads)
-    return dict(value=dof * 5 / wall, unit="DOF-stage/s", cores=threads, kind="port",
-                sample=f"1 CarpenterRK4-5 step (5 stages + CALCTIMESTEP) of the {nel}^3 N=7 workload, "
-                       f"{wall:.1f} s wall, OpenMP over elements",
-                single_core={"value": mesh1.nElems * 8 ** 3 * 5 / wall1, "unit": "DOF-stage/s",
-                             "sample": f"1 step of 8^3 elements, N=7, {wall1:.1f} s wall"})
+    variant, march = "native", "native"
+    try:
+        orc.load_native(tempfile.mkdtemp(prefix="fxg_oracle_"))
+    except Exception as exc:  # no compiler on this host: the portable -O3 build of the same sources
+        print(f"[bench] cpu baseline: native oracle build failed ({exc}); using liboracle_fast.so",
+              file=sys.stderr, flush=True)
+        variant, march = "fast", "x86-64-v3"
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def med(fn, reps=5):
+        fn()  # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    def case(N, node, split, riemann, nel_rhs, nel_step, steps_per_run, nthreads):
+        orc.set_threads(nthreads, variant)
+        prm = orc.params(N, node, split, riemann)
+        res = {}
+        for what, ne in (("rhs", nel_rhs), ("step", nel_step)):
+            mesh = periodic_box(ne, ne, ne)
+            m = orc.calc_metrics(N, node, mesh)
+            S = orc.Solver(prm, mesh, m, variant=variant)
+            U = testcase.tgv(m["Elem_xGP"])
+            dof = mesh.nElems * (N + 1) ** 3
+            if what == "rhs":
+                t = med(lambda: S.rhs(U))
+                res["rhs"] = {"value": dof / t, "unit": "DOF/s (one RHS)", "median_s": t, "mesh": f"{ne}^3"}
+            else:
+                cfl, dfl = fill_cfl_dfl(N, node, 0.8 if split else 0.9, 0.8 if split else 0.9)
+                R = orc.F(U.shape)
+                Uw = np.asfortranarray(U.copy())
+
+                def run():
+                    for _ in range(steps_per_run):
+                        dt, err, _ = S.calc_timestep(Uw, cfl, dfl)
+                        S.step(Uw, 0.0, dt, R)
+                t = med(run)
+                res["step"] = {"value": dof * 5 * steps_per_run / t, "unit": "DOF-stage/s", "median_s": t,
+                               "mesh": f"{ne}^3", "steps_per_run": steps_per_run}
+            del S
+        return res
+
+    c2 = case(7, orc.NODE_GL, 1, orc.RIEMANN_ROE, nel, 16, 1, threads)
+    c1 = case(3, orc.NODE_G, 0, orc.RIEMANN_LF, 8, 8, 50, threads)
+    one = case(7, orc.NODE_GL, 1, orc.RIEMANN_ROE, 8, 8, 1, 1)
+    orc.set_threads(threads, variant)
+    return dict(value=c2["step"]["value"], unit="DOF-stage/s", cores=threads, kind="port",
+                sample=f"config 2 scheme (N=7 GL split PI, Roe, BR1, TGV): CarpenterRK4-5 steps with CALCTIMESTEP "
+                       f"on a 16^3 sample, median of 5 after a warm-up, OpenMP over elements",
+                cpu_model=cpu_model(), nproc=os.cpu_count(), build=f"g++ -O3 -march={march} -fopenmp (oracle/Makefile)",
+                config2=c2, config1=c1,
+                single_core={"config2": one, "cores": 1})
 
 
 def main():
@@ -411,7 +453,8 @@ def main():
                            "unit": "GB/s", "frac": stage_achieved / HBM_PEAK_GBS,
                            "dof_stage_per_s_at_peak": HBM_PEAK_GBS * 1e9 / B_STAGE},
         "kernels": kern,
-        "rhs_only": {"value": r64["dof"] / r64["rhs_s"], "unit": "DOF/s (one RHS)", "ms": r64["rhs_s"] * 1e3},
+        "rhs_only": {"value": r64["dof"] / r64["rhs_s"], "unit": "DOF/s (one RHS)", "ms": r64["rhs_s"] * 1e3,
+                     "evaluations": RHS_EVALS},
         "tgv_analysis": {"ms": r64["analyze_s"] * 1e3, "n_analyze": 10, "ekin": r64["ekin"],
                          "case": "AnalyzeTestcase on the device (fxg_tgv_analyze), wall clock per call"},
     }
@@ -423,7 +466,9 @@ def main():
     if not args.no_adjoint:
         print("[bench] adjoint (TGV 16^3 N=5, 200 steps)", file=sys.stderr, flush=True)
         adj = adjoint_ratio()
+        # config 5 asks for stage checkpointing every 20 steps: that mode is the headline ratio
         out["adjoint_forward_ratio"] = adj["ratio"]
+        out["adjoint_forward_ratio_trajectory"] = adj["trajectory"]["ratio"]
         out["adjoint"] = adj
     else:
         out["adjoint_forward_ratio"] = None
@@ -437,7 +482,7 @@ def main():
         print("[bench] State file I/O", file=sys.stderr, flush=True)
         out["state_io"] = state_io(args.elems)
     if not args.no_cpu:
-        print("[bench] cpu baseline (oracle, one step)", file=sys.stderr, flush=True)
+        print("[bench] cpu baseline (oracle, native build, configs 1 and 2)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.elems)
     print(json.dumps(out), flush=True)
 
