@@ -33,13 +33,25 @@ namespace {
 
 // ---------------------------------------------------------------------------------------------
 // Kernel B: the rest of the stage.
-template <int N>
+// One LDS array, carved by phase (values in the working precision RF):
+//  staging to the gather of the surface terms: rows of np 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T, 7 E,
+//    then the neighbour face states (7 rows of NF: u, v, w, T, rho, p, E) and the surface
+//    contributions (5 rows of NF);
+//  direction loop: rows 0..5 as before, the metric terms Ja^1, Ja^2, Ja^3 of every node in rows
+//    6..14 (written once, from the registers the lifting loaded them into) and the contravariant
+//    viscous flux of the current direction in rows 15..18.
+// At N=7 that is 19 rows of 512 doubles (76 KB): with the side slots and the basis two workgroups
+// still share a CU's 160 KB, and the metric terms are read from memory once per stage. In FP32-bulk
+// (STAGE false) only the current direction's Ja^d is staged (rows 6..8, read from memory in each
+// direction, viscous flux in rows 9..12), so that four workgroups still share a CU.
+template <int N, bool STAGE>
 struct LdsB {
   static constexpr int np = D<N>::np, NF = D<N>::NF;
-  // sX holds the neighbour face states (7 rows of NF) and the surface contributions (5 rows of
-  // NF) until the face fluxes are gathered, then the direction loop's Ja^d (3 rows of np) and
-  // contravariant viscous fluxes (4 rows of np).
-  static constexpr int X = (7 * np > 12 * NF) ? 7 * np : 12 * NF;
+  static constexpr int FACE = 8 * np;                 // neighbour face states, then surface contributions
+  static constexpr int JA = 6 * np;                   // Ja^1..3 (9 rows), or Ja^d (3 rows)
+  static constexpr int FV = (STAGE ? 15 : 9) * np;    // contravariant viscous flux, 4 rows
+  static constexpr int A = 8 * np + 12 * NF, Bm = FV + 4 * np;
+  static constexpr int SIZE = A > Bm ? A : Bm;
 };
 
 // The Riemann solver is a template parameter: each instance then carries only its own solver's
@@ -47,8 +59,8 @@ struct LdsB {
 // In FP32-bulk the LDS arrays are FP32 (38 KB at N=7) and the kernel is held to 64 VGPRs, so
 // four workgroups share a CU (six waves per SIMD below N=7, where 8 would spill).
 // LAM: the last stage of a step, which also forms CALCTIMESTEP's per-element spectral radii of the
-// new state; in FP64 from the metric terms the direction loop loaded (the same values as the
-// time-step kernel reads), so they are not read from memory a second time.
+// new state; in FP64 from the metric terms staged in LDS (the same values as the time-step kernel
+// reads), so they are not read from memory a second time.
 template <int N, class RF, int SOLVER, bool BC, bool LAM>
 __global__ __launch_bounds__(D<N>::NT)
 __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
@@ -56,15 +68,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
                                                      Gas gas, double* Ut, RKStage st, RF* __restrict__ Gout,
                                                      const double* __restrict__ bcP) {
   constexpr int n = D<N>::n, nf = D<N>::nf, np = D<N>::np, NT = D<N>::NT, NF = D<N>::NF;
+  constexpr bool STAGE = sizeof(RF) == 8;  // all metric terms staged once (FP64)
+  using L = LdsB<N, STAGE>;
   // LDS holds the working precision RF: the values are rounded to RF where they are stored, as
   // they were rounded where they are read. sA rows: 0 rho, 1 p, 2 H, 3 u, 4 v, 5 w, 6 T, 7 E
-  __shared__ RF sA[8 * np];
-  __shared__ __align__(16) RF sX[LdsB<N>::X];
+  __shared__ __align__(16) RF sL[L::SIZE];
   __shared__ unsigned sIdx[NF];
   __shared__ double sB[3 * n * n + 4 * n];
-  // neighbour rows in sX: 0 u, 1 v, 2 w, 3 T, 4 rho, 5 p, 6 E; surface rows follow
-  RF* sNb = sX;
-  RF* sSurf = sX + 7 * NF;
+  RF* sA = sL;
+  // neighbour rows: 0 u, 1 v, 2 w, 3 T, 4 rho, 5 p, 6 E; surface rows follow
+  RF* sNb = sL + L::FACE;
+  RF* sSurf = sNb + 7 * NF;
   const BasisLayout B{n};
   const GasT<RF> gt(gas);
   const int e = blockIdx.x, t = threadIdx.x;
@@ -165,9 +179,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
   const int sides = act ? node_sides<N>(i, j, k) : 0;
   RF acc[5] = {0, 0, 0, 0, 0};
   RF fv[12];  // contravariant viscous flux Ja^d . (f,g,h)_v, d-major
+  // FP64: the node's metric terms, read once, held until the surface terms are gathered and then
+  // staged in LDS for the direction loop
+  RF JaK[STAGE ? 9 : 1] = {};
+  double sjd = 0.0;
   if (act) {
     RF Ja[9], sj;
     load_metrics<RF>(m, gid, Ja, sj);
+    if constexpr (STAGE) {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) JaK[r] = Ja[r];
+    }
     RF g[12];
     lift_node<N, RF>(t, sA + 3 * np, sB, sNb, Ja, sj, g);  // u,v,w,T rows
     if (Gout) {  // (np, 12, nElems) like the four-kernel path
@@ -201,33 +223,33 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
       for (int v = 0; v < 5; ++v) acc[v] += (RF)sSurf[v * NF + x];
     }
   }
-  __syncthreads();  // sX is reused by the direction loop
+  __syncthreads();  // the face rows are free: the metric terms take their place
 
   // split-form volume integral with the viscous weak-form part (volint.f90:211-371)
   const double* DV = sB + B.DVolSurf();
   const double* DH = sB + B.D_Hat_T();
   RF rho = 0, uu = 0, vv = 0, ww = 0, pp = 0, HH = 0;
+  RF* sJa = sL + L::JA;
+  RF* sF = sL + L::FV;
   if (act) {
     rho = (RF)sA[t]; pp = (RF)sA[np + t]; HH = (RF)sA[2 * np + t];
     uu = (RF)sA[3 * np + t]; vv = (RF)sA[4 * np + t]; ww = (RF)sA[5 * np + t];
+    if constexpr (STAGE) {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) sJa[r * np + t] = JaK[r];
+    }
   }
-  RF* sM = sX;
-  RF* sF = sX + 3 * np;
   // the register R and the state U of the update are loaded before the last direction (whose
   // arithmetic then hides their latency; fv of the first two directions no longer holds registers)
   double rpre[5] = {0, 0, 0, 0, 0}, upre[5] = {0, 0, 0, 0, 0};
-  constexpr bool keepJ = LAM && sizeof(RF) == 8;
-  double Jk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // Ja^1..3 of the node for the spectral radii (keepJ)
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    RF m0 = 0, m1 = 0, m2 = 0;
+    RF* sM = sJa + (STAGE ? 3 * d * np : 0);
     if (act) {
-      const double* Jg = (d == 0 ? m.Mf : (d == 1 ? m.Mg : m.Mh)) + 3 * gid;
-      m0 = (RF)Jg[0]; m1 = (RF)Jg[1]; m2 = (RF)Jg[2];
-      if constexpr (keepJ) {
-        Jk[3 * d] = (double)m0; Jk[3 * d + 1] = (double)m1; Jk[3 * d + 2] = (double)m2;
+      if constexpr (!STAGE) {
+        const double* Jg = (d == 0 ? m.Mf : (d == 1 ? m.Mg : m.Mh)) + 3 * gid;
+        sM[t] = (RF)Jg[0]; sM[np + t] = (RF)Jg[1]; sM[2 * np + t] = (RF)Jg[2];
       }
-      sM[t] = m0; sM[np + t] = m1; sM[2 * np + t] = m2;
 #pragma unroll
       for (int c = 0; c < 4; ++c) sF[c * np + t] = fv[4 * d + c];
       if (d == 2 && st.mode != 0) {
@@ -240,6 +262,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
     }
     __syncthreads();
     if (act) {
+      const RF m0 = sM[t], m1 = sM[np + t], m2 = sM[2 * np + t];
       const int me = d == 0 ? i : (d == 1 ? j : k);
       const int stride = d == 0 ? 1 : (d == 1 ? n : n * n);
       const int base = t - me * stride;
@@ -272,7 +295,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
   // Ut = -Ut*sJ and the register update; with st.lamSum also CALCTIMESTEP's per-element spectral
   // radii of the new state (the time-step kernel's arithmetic, dt_node_lambda)
   double un[5] = {0, 0, 0, 0, 0};
-  double sjd = 0.0;
   if (act) {
     sjd = m.sJ[gid];
     double ut[5];
@@ -301,16 +323,22 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
       }
     }
   }
-  if constexpr (LAM) {  // the direction loop's last barrier freed sX
+  if constexpr (LAM) {  // the direction loop's last barrier freed the viscous-flux rows
     double lam[6] = {0, 0, 0, 0, 0, 0};
     int isnan_rho = 0;
     if (act) {
-      if constexpr (keepJ) dt_node_lambda(un, gas, sjd, Jk, Jk + 3, Jk + 6, lam, isnan_rho);
-      else dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
+      if constexpr (sizeof(RF) == 8) {
+        double Jk[9];  // Ja^1..3 of the node as staged (FP64: the values the time-step kernel reads)
+#pragma unroll
+        for (int r = 0; r < 9; ++r) Jk[r] = (double)sJa[r * np + t];
+        dt_node_lambda(un, gas, sjd, Jk, Jk + 3, Jk + 6, lam, isnan_rho);
+      } else {
+        dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
+      }
     }
     double mx[6];
     int nanr = 0;
-    double* sRd = reinterpret_cast<double*>(sX);
+    double* sRd = reinterpret_cast<double*>(sF);
     block_lambda_max<NT>(lam, isnan_rho, sRd, reinterpret_cast<int*>(sRd + 6 * (NT / 64)), t, mx, nanr);
     if (t == 0) {
       st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
