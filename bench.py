@@ -61,7 +61,7 @@ def make_solver(nel, precision):
     return mesh, s
 
 
-def run_gpu(nel, precision, steps, warmup, profile):
+def run_gpu(nel, precision, steps, warmup, profile, rhs_evals=RHS_EVALS):
     mesh, s = make_solver(nel, precision)
     U0 = s_tgv(s)
     s.set_state(U0)
@@ -75,14 +75,14 @@ def run_gpu(nel, precision, steps, warmup, profile):
     s.sync()
     wall = time.perf_counter() - t0
     kt = s.kernel_times() if profile else None
-    # RHS-only rate (DGTimeDerivative_weakForm alone), 1000 evaluations as the configuration asks
+    # RHS-only rate (DGTimeDerivative_weakForm alone), 1000 evaluations
     s.profile(False)
     s.sync()
     r0 = time.perf_counter()
-    for _ in range(RHS_EVALS):
+    for _ in range(rhs_evals):
         s.DGTimeDerivative_weakForm(t)
     s.sync()
-    rhs_wall = (time.perf_counter() - r0) / RHS_EVALS
+    rhs_wall = (time.perf_counter() - r0) / rhs_evals
     # the TGV testcase analysis on the device (NAnalyze = 10 as in the TGV parameter files), after
     # the right-hand sides above, i.e. without a recomputation of the last stage's gradients
     a0 = time.perf_counter()
@@ -139,7 +139,7 @@ def fp32_accuracy():
 def adjoint_ratio():
     """Gradient of the kinetic energy at T w.r.t. U(0): TGV 16^3, N=5 Gauss-Lobatto split PI, Roe,
     200 steps (dt frozen from the primal run). `ratio` = time of the adjoint (backward) call / time
-    of the 200 primal forward steps, with checkpoints every 20 steps (config 5), the backward
+    of the 200 primal forward steps, with checkpoints every 20 steps, the backward
     recomputing the stage states of each segment; `trajectory` = the same with the whole trajectory
     stored by the forward in HBM (71 GB: every stage state and its face states), no recomputation."""
     import torch
@@ -187,7 +187,7 @@ def adjoint_ratio():
     state = U0.nbytes
     face_states = 10 * (5 + 1) ** 2 * mesh.nSides * 8  # both slots of every side, 5 doubles per point
     s.close()
-    return {"ratio": t_bwd / t_fwd, "mode": "checkpoints every 20 steps (config 5), segments recomputed",
+    return {"ratio": t_bwd / t_fwd, "mode": "checkpoints every 20 steps, segments recomputed",
             "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "adjoint_s": t_bwd,
             "gradient_total_ratio": (t_fwd_ck + t_bwd) / t_fwd,
             "memory_bytes": {"checkpoints": (len(dts) // every) * state, "segment_stage_states": every * 5 * state,
@@ -310,8 +310,8 @@ def cpu_baseline(nel):
     parity build is separate) and timed on its cores: after one warm-up, the median of 5 runs of
     each case. Config 2 (N=7 GL split PI, Roe, BR1, TGV): the right-hand side alone on the full
     nel^3 mesh, and whole CarpenterRK4-5 steps (CALCTIMESTEP + 5 stages) on a 16^3 sample of the
-    same scheme; config 1 (8^3, N=3 Gauss, weak form, LF, BR1): the same two figures, steps in runs
-    of 50 as the config. Also on one core (config 2 on an 8^3 sample). DOF-stage/s."""
+    same scheme; the plain method (TGV 8^3, N=3 Gauss, weak form, LF, BR1): the same two figures,
+    steps in runs of 50. Also on one core (the N=7 scheme on an 8^3 sample). DOF-stage/s."""
     import statistics
     import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -370,11 +370,11 @@ def cpu_baseline(nel):
     one = case(7, orc.NODE_GL, 1, orc.RIEMANN_ROE, 8, 8, 1, 1)
     orc.set_threads(threads, variant)
     return dict(value=c2["step"]["value"], unit="DOF-stage/s", cores=threads, kind="port",
-                sample=f"config 2 scheme (N=7 GL split PI, Roe, BR1, TGV): CarpenterRK4-5 steps with CALCTIMESTEP "
+                sample=f"TGV, N=7 GL split PI, Roe, BR1: CarpenterRK4-5 steps with CALCTIMESTEP "
                        f"on a 16^3 sample, median of 5 after a warm-up, OpenMP over elements",
                 cpu_model=cpu_model(), nproc=os.cpu_count(), build=f"g++ -O3 -march={march} -fopenmp (oracle/Makefile)",
-                config2=c2, config1=c1,
-                single_core={"config2": one, "cores": 1})
+                tgv_n7_split=c2, tgv8_n3_weak_lf=c1,
+                single_core={"tgv_n7_split": one, "cores": 1})
 
 
 def main():
@@ -383,6 +383,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--elems", type=int, default=32)
+    ap.add_argument("--rhs-evals", type=int, default=RHS_EVALS,
+                    help="right-hand sides timed alone after the steps (default 1000)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -414,7 +416,7 @@ def main():
 
     print(f"[bench] fp64 {args.elems}^3 N=7: {args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr,
           flush=True)
-    r64 = run_gpu(args.elems, "fp64", args.steps, args.warmup, profile=True)
+    r64 = run_gpu(args.elems, "fp64", args.steps, args.warmup, profile=True, rhs_evals=args.rhs_evals)
     stages = 5
     value = r64["dof"] * stages * args.steps / r64["wall"]
     ms_per_step = 1e3 * r64["wall"] / args.steps
@@ -454,19 +456,19 @@ def main():
                            "dof_stage_per_s_at_peak": HBM_PEAK_GBS * 1e9 / B_STAGE},
         "kernels": kern,
         "rhs_only": {"value": r64["dof"] / r64["rhs_s"], "unit": "DOF/s (one RHS)", "ms": r64["rhs_s"] * 1e3,
-                     "evaluations": RHS_EVALS},
+                     "evaluations": args.rhs_evals},
         "tgv_analysis": {"ms": r64["analyze_s"] * 1e3, "n_analyze": 10, "ekin": r64["ekin"],
                          "case": "AnalyzeTestcase on the device (fxg_tgv_analyze), wall clock per call"},
     }
     if not args.no_fp32:
         print("[bench] fp32-bulk", file=sys.stderr, flush=True)
-        r32 = run_gpu(args.elems, "fp32-bulk", args.steps, args.warmup, profile=False)
+        r32 = run_gpu(args.elems, "fp32-bulk", args.steps, args.warmup, profile=False, rhs_evals=args.rhs_evals)
         out["fp32_bulk"] = {"value": r32["dof"] * stages * args.steps / r32["wall"], "unit": "DOF-stage/s",
                             "ms_per_step": 1e3 * r32["wall"] / args.steps, **fp32_accuracy()}
     if not args.no_adjoint:
         print("[bench] adjoint (TGV 16^3 N=5, 200 steps)", file=sys.stderr, flush=True)
         adj = adjoint_ratio()
-        # config 5 asks for stage checkpointing every 20 steps: that mode is the headline ratio
+        # the headline ratio is the checkpointed mode (every 20 steps), the trajectory mode is secondary
         out["adjoint_forward_ratio"] = adj["ratio"]
         out["adjoint_forward_ratio_trajectory"] = adj["trajectory"]["ratio"]
         out["adjoint"] = adj
@@ -482,7 +484,7 @@ def main():
         print("[bench] State file I/O", file=sys.stderr, flush=True)
         out["state_io"] = state_io(args.elems)
     if not args.no_cpu:
-        print("[bench] cpu baseline (oracle, native build, configs 1 and 2)", file=sys.stderr, flush=True)
+        print("[bench] cpu baseline (oracle, native build)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.elems)
     print(json.dumps(out), flush=True)
 

@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
     L.fxg_create_from_nodes.argtypes = [C.POINTER(FxgParams), C.c_int, C.c_int, vp, vp, C.c_int, vp, C.POINTER(vp)]
     L.fxg_destroy.argtypes = [vp]
     L.fxg_set_boundaries.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp]
+    L.fxg_set_boundary_data.argtypes = [vp, vp]
     L.fxg_set_state.argtypes = [vp, vp]
     L.fxg_get_state.argtypes = [vp, vp]
     L.fxg_get_ut.argtypes = [vp, vp]
@@ -86,6 +87,7 @@ def lib() -> C.CDLL:
 
 # Every exported symbol of include/fxg.h (checked by the CPU test suite).
 EXPORTS = ["fxg_last_error", "fxg_version", "fxg_create", "fxg_create_from_nodes", "fxg_destroy", "fxg_set_boundaries",
+           "fxg_set_boundary_data",
            "fxg_set_state",
            "fxg_get_state", "fxg_get_ut", "fxg_get_gradients", "fxg_tgv_analyze", "fxg_get_metrics",
            "fxg_dg_time_derivative",

@@ -187,6 +187,9 @@ struct fxg_ctx {
   // boundary sides 1..nBC: their conditions must be set (fxg_set_boundaries) before a right-hand side
   int nBC = 0;
   bool bcSet = false;
+  // types 12/121 take BCDataPrim(6,0:N,0:N,nBC) from the host (fxg_set_boundary_data)
+  bool bcNeedsData = false;
+  double* bcData = nullptr;
   int* bcInfo = nullptr;
   double* refPrim = nullptr;
 };
@@ -304,7 +307,7 @@ int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, con
     if (alloc_owned(c, &c->fw.FU[0], 10 * face) || alloc_owned(c, &c->fw.FU[1], 10 * face) ||
         alloc_owned(c, &FG, 8 * face * gsz) || alloc_owned(c, &c->fw.Jsoa, fxg::packed_metrics_size(c->N, K)) ||
         alloc_owned(c, &c->lamSum, 2 * (size_t)K) || alloc_owned(c, &c->lamNan, (size_t)K) ||
-        alloc_owned(c, &c->fw.bcP, (size_t)c->nf * nBC))
+        alloc_owned(c, &c->fw.bcP, 4 * (size_t)c->nf * nBC))
       return 1;
     char* gBC;
     if (alloc_owned(c, &gBC, 12 * (size_t)c->nf * nBC * gsz)) return 1;
@@ -344,6 +347,8 @@ void ctx_free(fxg_ctx* c) {
 // FU[cur] already holds the face states of U (written by the previous stage).
 int rhs(fxg_ctx* c, const double* U, double* Ut, const fxg::RKStage& st, hipStream_t s, bool faces_fresh = false) {
   if (c->nBC > 0 && !c->bcSet) return fail("the mesh has boundary sides: set their conditions with fxg_set_boundaries");
+  if (c->bcNeedsData && !c->bcData)
+    return fail("boundary types 12/121 need their states: set BCDataPrim with fxg_set_boundary_data");
   if (c->fused) {
     HIPCHECK(fxg::launch_rhs_fused(c->m, c->fw, c->gas, c->p.riemann, c->p.precision == FXG_PREC_FP32_BULK, U, Ut, st,
                                    faces_fresh, c->cur, st.mode == 0 ? c->w.G : nullptr, s, c->timer));
@@ -486,10 +491,18 @@ int fxg_create_from_nodes(const fxg_params* p, int nElems, int nSides, const int
   double* dNC = nullptr;
   if (upload(c, &dNC, NodeCoords, 3 * ng * nElems)) { ctx_free(c); return 1; }
   std::string err;
+  // four-kernel path: each side of a face takes its own face metric too (the slave's in slaveNS,
+  // its side fluxes in FluxS), as the fused path does with its boundary nodes
+  double* slaveNS = nullptr;
+  if (!c->fused) {
+    const size_t face = (size_t)c->nf * nSides;
+    if (alloc_owned(c, &slaveNS, 3 * face) || alloc_owned(c, &c->w.FluxS, 5 * face)) { ctx_free(c); return 1; }
+  }
   if (fxg::compute_metrics_gpu(c->N, p->nodeType, NGeo, nElems, dNC, c->m, c->Mf, c->Mg, c->Mh, c->sJ, c->xGP, c->NV,
-                               c->T1, c->T2, c->SE, c->stream, err)) {
+                               c->T1, c->T2, c->SE, slaveNS, c->stream, err)) {
     fail(err); ctx_free(c); return 1;
   }
+  c->m.slaveNS = slaveNS;
   // the curl-form face metric depends only on the face, so each element's own boundary metric
   // is the face metric; using it on both sides keeps the freestream exact to flux round-off
   c->m.ownFaceMetric = 1;
@@ -513,19 +526,34 @@ int fxg_set_boundaries(fxg_ctx* c, int nBCs, const int* BC, const int* BoundaryT
   if (!BC || !BoundaryType || nBCs < 1) return fail("BC and BoundaryType are required for a mesh with boundary sides");
   if (nRefState < 0 || (nRefState > 0 && !RefStatePrim)) return fail("RefStatePrim is NULL");
   std::vector<int> info(2 * (size_t)c->nBC);
+  bool needsData = false;
   for (int s = 0; s < c->nBC; ++s) {
     const int iBC = BC[s];
     if (iBC < 1 || iBC > nBCs) return fail("BC(SideID) out of range 1..nBCs");
     const int type = BoundaryType[iBC - 1], state = BoundaryType[iBC - 1 + nBCs];
-    if (type != 2 && type != 3 && type != 4 && type != 9)
-      return fail("boundary type " + std::to_string(type) + " is not supported (2, 3, 4 and 9 are)");
-    if ((type == 2 || type == 4) && (state < 1 || state > nRefState))
+    const bool known = type == 2 || type == 3 || type == 4 || type == 9 || type == 91 || type == 12 || type == 121 ||
+                       type == 23 || type == 24 || type == 25 || type == 27;
+    if (!known)
+      return fail("boundary type " + std::to_string(type) + " is not supported (2, 3, 4, 9, 12, 23, 24, 25, 27, 91 "
+                  "and 121 are; 22, an exact function of time, is evaluated by the host as 121 at each call time)");
+    // the types that read RefStatePrim(:,BCState) (InitBC, getboundaryflux.f90:64-218)
+    const bool usesRef = type == 2 || type == 4 || type == 23 || type == 24 || type == 25 || type == 27;
+    if (usesRef && (state < 1 || state > nRefState))
       return fail("boundary type " + std::to_string(type) + " needs a RefState index 1..nRefState as its state "
                   "(state 0, an exact function, is not supported)");
-    if (type == 4 && !(RefStatePrim[6 * (state - 1) + 5] > 0.0))
-      return fail("an isothermal wall needs a positive wall temperature");
+    const double* ref = usesRef ? RefStatePrim + 6 * (state - 1) : nullptr;
+    if (type == 4 && !(ref[5] > 0.0)) return fail("an isothermal wall needs a positive wall temperature");
+    if ((type == 24 || type == 25) && !(ref[4] > 0.0)) return fail("an outflow boundary needs a positive pressure");
+    if (type == 23 && !(ref[1] >= 0.0)) return fail("an outflow Mach number must not be negative");
+    if (type == 27) {
+      const double a2 = ref[1] * ref[1] + ref[2] * ref[2] + ref[3] * ref[3];
+      if (!(ref[0] > 0.0) || !(ref[4] > 0.0) || !(std::fabs(a2 - 1.0) < 1e-12))
+        return fail("a subsonic inflow (27) needs RefStatePrim = (Tt>0, unit direction a(1:3), pt>0), as InitBC "
+                    "leaves it");
+    }
+    needsData = needsData || type == 12 || type == 121;
     info[2 * s] = type;
-    info[2 * s + 1] = (type == 2 || type == 4) ? state - 1 : -1;
+    info[2 * s + 1] = usesRef ? state - 1 : -1;
   }
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -540,7 +568,24 @@ int fxg_set_boundaries(fxg_ctx* c, int nBCs, const int* BC, const int* BoundaryT
   c->m.nBC = c->nBC;
   c->m.bcInfo = c->bcInfo;
   c->m.refPrim = c->refPrim;
+  c->bcNeedsData = needsData;
   c->bcSet = true;
+  return 0;
+}
+
+int fxg_set_boundary_data(fxg_ctx* c, const double* BCDataPrim) {
+  if (!c) return fail("ctx is NULL");
+  if (c->nBC == 0) return 0;
+  if (!BCDataPrim) return fail("BCDataPrim is NULL");
+  const size_t n = 6 * (size_t)c->nf * c->nBC;
+  for (size_t x = 0; x < n; x += 6)
+    if (!(BCDataPrim[x] > 0.0) || !(BCDataPrim[x + 4] > 0.0) || !(BCDataPrim[x + 5] > 0.0))
+      return fail("BCDataPrim needs positive density, pressure and temperature at every boundary point");
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (!c->bcData && alloc_owned(c, &c->bcData, n)) return 1;
+  HIPCHECK(hipMemcpy(c->bcData, BCDataPrim, n * sizeof(double), hipMemcpyHostToDevice));
+  c->m.bcData = c->bcData;
   return 0;
 }
 

@@ -155,9 +155,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
     if constexpr (BC) {
       // a wall side: the inviscid flux is (0, p_wall n, 0) (getboundaryflux.f90:580-592)
       const int side = (int)(own / (2u * (unsigned)nf));
-      if (side < m.nBC && m.bcInfo[2 * side] != BC_DIRICHLET) {
+      if (side < m.nBC && !bc_riemann_type(m.bcInfo[2 * side])) {
         wall = true;
-        F[0] = RF(0); F[1] = (RF)bcP[slot_side_point<N>(own)]; F[2] = F[3] = F[4] = RF(0);
+        F[0] = RF(0); F[1] = (RF)bcP[4 * slot_side_point<N>(own)]; F[2] = F[3] = F[4] = RF(0);
       }
     }
     if (!wall) {
@@ -397,7 +397,7 @@ hipError_t fused_n(const DevMesh& m, const FusedWork& w, const Gas& gas, int rie
       const hipError_t err = launch_bc_ghost(m, w.FU[cur], w.bcP, gas, s);
       if (err != hipSuccess) return err;
     }
-    const hipError_t err = launch_visc_faces(m, sizeof(RF) == 4, U, w.Jsoa, w.FU[cur], w.FG, gas, w.gBC, s);
+    const hipError_t err = launch_visc_faces(m, sizeof(RF) == 4, U, w.Jsoa, w.FU[cur], w.FG, gas, w.gBC, w.bcP, s);
     if (err != hipSuccess) return err;
   }
   {

@@ -39,11 +39,16 @@ struct DevMesh {
   // terms at the face node (metrics computed by the library), 0 = the side's NormVec/SurfElem as
   // supplied by the host, shared by master and slave as FLEXI does
   int ownFaceMetric;
+  // four-kernel path with metrics computed by the library: the slave element's own n*SurfElem of
+  // each side point, (3,0:N,0:N,nSides), oriented as the master's normal; null otherwise
+  const double* slaveNS = nullptr;
   // boundary sides 0..nBC-1 (FLEXI numbers them first): per side (BC type, RefState index or -1),
   // and RefStatePrim(6,nRefState)
   int nBC = 0;
   const int* bcInfo = nullptr;
   const double* refPrim = nullptr;
+  // BCDataPrim(6,0:N,0:N,nBC) of the boundaries of types 12 and 121 (host-supplied), or null
+  const double* bcData = nullptr;
   // face_slot of every own face point of every element, (6*(N+1)^2, nElems), or null (then
   // formed from ElemToSide and the inverse S2V2 map)
   const unsigned* slotTab = nullptr;
@@ -58,6 +63,7 @@ struct DevWork {
   void* Gf_m;      // face gradients (12,0:N,0:N,nSides)
   void* Gf_s;
   double* Flux;    // Flux_master(5,0:N,0:N,nSides); Flux_slave equals it (fillflux.f90:199)
+  double* FluxS = nullptr;  // with DevMesh::slaveNS: Flux_slave with the slave's own face metric
   double* bcLift;  // lifting flux of the boundary sides before SurfElem, (4,0:N,0:N,nBC)
   double* dtElem;  // (nElems)
   int* errElem;    // (nElems)
@@ -101,7 +107,9 @@ struct FusedWork {
   double* FU[2];  // face states, 5 per side point and slot, double-buffered across stages
   void* FG;       // viscous normal face fluxes f_v.(n SurfElem), 4 per side point and slot
   double* Jsoa;   // metric terms of the boundary nodes, rows of NB per element: Ja^1..3 x,y,z, sJ
-  double* bcP = nullptr;  // wall pressure (PRESSURE_RIEMANN) per boundary side point, (0:N,0:N,nBC)
+  // per boundary side point, (4,0:N,0:N,nBC): walls the pressure (PRESSURE_RIEMANN) first, Riemann-type
+  // boundaries the boundary state's velocity in 1..3
+  double* bcP = nullptr;
   void* gBC = nullptr;    // lifted gradients at the boundary side points, 12 per point (gradient type)
 };
 // Doubles of FusedWork::Jsoa.
@@ -118,10 +126,11 @@ bool fused_supported(int N, int nodeType, int split, long long nSides);
 // G (optional, (np,12,nElems) of the gradient type) receives the lifted gradients.
 // Kernel A of the fused stage (fxg_visc.hip): viscous normal face fluxes into FG (float if fp32).
 hipError_t launch_visc_faces(const DevMesh& m, bool fp32, const double* U, const double* Jsoa, const double* FU,
-                             void* FG, const Gas& gas, void* gBC, hipStream_t s);
+                             void* FG, const Gas& gas, void* gBC, const double* bcP, hipStream_t s);
 // Boundary sides on the fused path (fxg_visc.hip): the outer slot of each boundary side point gets a
 // face state whose BR1 lifting difference with the inner state is the boundary lifting flux (and,
-// for Dirichlet sides, whose Riemann flux is the boundary flux); bcP the walls' pressure.
+// for Riemann-type sides, whose Riemann flux is the boundary flux); bcP the walls' pressure and the
+// Riemann-type boundary states' velocity.
 hipError_t launch_bc_ghost(const DevMesh& m, double* FU, double* bcP, const Gas& gas, hipStream_t s);
 hipError_t launch_rhs_fused(const DevMesh& m, const FusedWork& w, const Gas& gas, int riemann, int fp32,
                             const double* U, double* Ut, const RKStage& st, bool faces_fresh, int cur,

@@ -231,29 +231,13 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_lift(const double* __restrict__ 
 template <class RF, bool SPLIT>
 __global__ __launch_bounds__(256) void k_flux(DevMesh m, const double* __restrict__ Um, const double* __restrict__ Us,
                                             const RF* __restrict__ Gm, const RF* __restrict__ Gs, Gas gas,
-                                            int solver, double* __restrict__ Flux, int first, int nPts) {
+                                            int solver, double* __restrict__ Flux, double* __restrict__ FluxS,
+                                            int first, int nPts) {
   const int fp = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (fp >= nPts) return;
   const GasT<RF> g(gas);
   const Prim pl = cons_to_prim(Um + 5 * (size_t)fp, gas);
   const Prim pr = cons_to_prim(Us + 5 * (size_t)fp, gas);
-  RF nv[3], t1[3], t2[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    nv[c] = (RF)m.NormVec[3 * (size_t)fp + c];
-    t1[c] = (RF)m.TangVec1[3 * (size_t)fp + c];
-    t2[c] = (RF)m.TangVec2[3 * (size_t)fp + c];
-  }
-  const RotState<RF> L = rotate<RF>((RF)pl.rho, (RF)Um[5 * (size_t)fp + 4], (RF)pl.p, (RF)pl.u, (RF)pl.v, (RF)pl.w, nv, t1, t2);
-  const RotState<RF> R = rotate<RF>((RF)pr.rho, (RF)Us[5 * (size_t)fp + 4], (RF)pr.p, (RF)pr.u, (RF)pr.v, (RF)pr.w, nv, t1, t2);
-  RF F[5];
-  riemann_rot<RF>(g, solver, SPLIT, L, R, F);
-  RF Fo[5];
-  Fo[0] = F[0];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) Fo[1 + c] = nv[c] * F[1] + t1[c] * F[2] + t2[c] * F[3];
-  Fo[4] = F[4];
-  // ViscousFlux_Side (riemann.f90:448-498): 0.5*(f_L + f_R).n
   RF gl[12], gr[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) {
@@ -263,12 +247,55 @@ __global__ __launch_bounds__(256) void k_flux(DevMesh m, const double* __restric
   RF fxl[4], fyl[4], fzl[4], fxr[4], fyr[4], fzr[4];
   diff_flux<RF>(g, (RF)pl.u, (RF)pl.v, (RF)pl.w, gl, gl + 4, gl + 8, fxl, fyl, fzl);
   diff_flux<RF>(g, (RF)pr.u, (RF)pr.v, (RF)pr.w, gr, gr + 4, gr + 8, fxr, fyr, fzr);
+  // the side's flux with the master's face metric (FLEXI's Flux_master = Flux_slave) and, with
+  // metrics computed by the library, once more with the slave element's own face metric
+  const int nFlux = m.slaveNS ? 2 : 1;
+  for (int side = 0; side < nFlux; ++side) {
+    RF nv[3], t1[3], t2[3];
+    double se;
+    if (side == 0) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    Fo[1 + k] = Fo[1 + k] + RF(0.5) * (nv[0] * (fxl[k] + fxr[k]) + nv[1] * (fyl[k] + fyr[k]) + nv[2] * (fzl[k] + fzr[k]));
-  const double se = m.SurfElem[fp];
+      for (int c = 0; c < 3; ++c) {
+        nv[c] = (RF)m.NormVec[3 * (size_t)fp + c];
+        t1[c] = (RF)m.TangVec1[3 * (size_t)fp + c];
+        t2[c] = (RF)m.TangVec2[3 * (size_t)fp + c];
+      }
+      se = m.SurfElem[fp];
+    } else {
+      const double* ns = m.slaveNS + 3 * (size_t)fp;
+      se = sqrt(ns[0] * ns[0] + ns[1] * ns[1] + ns[2] * ns[2]);
+      nv[0] = (RF)(ns[0] / se); nv[1] = (RF)(ns[1] / se); nv[2] = (RF)(ns[2] / se);
+      // tangential basis: Gram-Schmidt of the coordinate axis least aligned with n
+      const RF ax = fabs(nv[0]), ay = fabs(nv[1]), az = fabs(nv[2]);
+      RF h[3] = {RF(0), RF(0), RF(0)};
+      if (ax <= ay && ax <= az) h[0] = RF(1); else if (ay <= az) h[1] = RF(1); else h[2] = RF(1);
+      const RF dt_ = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
+      RF tt[3] = {h[0] - dt_ * nv[0], h[1] - dt_ * nv[1], h[2] - dt_ * nv[2]};
+      const RF tn = sqrt(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
+      t1[0] = tt[0] / tn; t1[1] = tt[1] / tn; t1[2] = tt[2] / tn;
+      t2[0] = nv[1] * t1[2] - nv[2] * t1[1];
+      t2[1] = nv[2] * t1[0] - nv[0] * t1[2];
+      t2[2] = nv[0] * t1[1] - nv[1] * t1[0];
+    }
+    const RotState<RF> L = rotate<RF>((RF)pl.rho, (RF)Um[5 * (size_t)fp + 4], (RF)pl.p, (RF)pl.u, (RF)pl.v, (RF)pl.w,
+                                      nv, t1, t2);
+    const RotState<RF> R = rotate<RF>((RF)pr.rho, (RF)Us[5 * (size_t)fp + 4], (RF)pr.p, (RF)pr.u, (RF)pr.v, (RF)pr.w,
+                                      nv, t1, t2);
+    RF F[5];
+    riemann_rot<RF>(g, solver, SPLIT, L, R, F);
+    RF Fo[5];
+    Fo[0] = F[0];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) Flux[5 * (size_t)fp + v] = (double)Fo[v] * se;
+    for (int c = 0; c < 3; ++c) Fo[1 + c] = nv[c] * F[1] + t1[c] * F[2] + t2[c] * F[3];
+    Fo[4] = F[4];
+    // ViscousFlux_Side (riemann.f90:448-498): 0.5*(f_L + f_R).n
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      Fo[1 + k] = Fo[1 + k] + RF(0.5) * (nv[0] * (fxl[k] + fxr[k]) + nv[1] * (fyl[k] + fyr[k]) + nv[2] * (fzl[k] + fzr[k]));
+    double* dst = side == 0 ? Flux : FluxS;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) dst[5 * (size_t)fp + v] = (double)Fo[v] * se;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -286,7 +313,8 @@ __device__ __forceinline__ int bc_point(const DevMesh& m, const double* __restri
   }
   const int side = fp / ((m.N + 1) * (m.N + 1));
   const int type = m.bcInfo[2 * side], st = m.bcInfo[2 * side + 1];
-  boundary_state(gas, type, st >= 0 ? m.refPrim + 6 * st : nullptr, Pm, nv, t1, t2, Pb);
+  boundary_state(gas, type, st >= 0 ? m.refPrim + 6 * st : nullptr, m.bcData ? m.bcData + 6 * (size_t)fp : nullptr,
+                 Pm, nv, t1, t2, Pb);
   return type;
 }
 
@@ -304,9 +332,9 @@ __global__ __launch_bounds__(256) void k_bc_lift(DevMesh m, const double* __rest
 }
 
 // GetBoundaryFlux (getboundaryflux.f90:480-789) times SurfElem (fillflux.f90:192-199): Riemann
-// solver against the boundary state plus ViscousFlux for Dirichlet sides; for walls the Euler
-// flux (0, p n, 0) plus the viscous flux of the wall state with the inner gradients (without
-// their normal part at a slip wall; no heat flux at an adiabatic wall).
+// solver against the boundary state plus ViscousFlux for Riemann-type sides; for walls the Euler
+// flux (0, p n, 0) plus the viscous flux of the wall state with the inner gradients (slip walls:
+// as slip_wall_gradients leaves them; no heat flux at an adiabatic wall).
 template <class RF>
 __global__ __launch_bounds__(256) void k_bc_flux(DevMesh m, const double* __restrict__ Um, const RF* __restrict__ Gm,
                                                Gas gas, int solver, int split, double* __restrict__ Flux, int nPts) {
@@ -317,7 +345,7 @@ __global__ __launch_bounds__(256) void k_bc_flux(DevMesh m, const double* __rest
 #pragma unroll
   for (int c = 0; c < 12; ++c) q[c] = (double)Gm[12 * (size_t)fp + c];
   const GasT<double> g(gas);
-  if (type == BC_DIRICHLET) {
+  if (bc_riemann_type(type)) {
     // both states' conservative variables from the primitive ones (PrimToCons), as GetBoundaryFlux
     const double Em = gas.sKappaM1 * Pm[4] + 0.5 * ((Pm[1] * Pm[0]) * Pm[1] + (Pm[2] * Pm[0]) * Pm[2] + (Pm[3] * Pm[0]) * Pm[3]);
     const double Eb = gas.sKappaM1 * Pb[4] + 0.5 * ((Pb[1] * Pb[0]) * Pb[1] + (Pb[2] * Pb[0]) * Pb[2] + (Pb[3] * Pb[0]) * Pb[3]);
@@ -340,20 +368,7 @@ __global__ __launch_bounds__(256) void k_bc_flux(DevMesh m, const double* __rest
 #pragma unroll
     for (int d = 0; d < 3; ++d) F[1 + d] = Pb[4] * nv[d];
     F[4] = 0.0;
-    if (type == BC_WALL_SLIP) {  // (I - n n^T) applied to every lifted gradient
-      double M[3][3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) M[a][b] = a == b ? 1.0 - nv[a] * nv[a] : -nv[a] * nv[b];
-      double r[12];
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int a = 0; a < 3; ++a) r[4 * a + v] = M[a][0] * q[v] + M[a][1] * q[4 + v] + M[a][2] * q[8 + v];
-#pragma unroll
-      for (int c = 0; c < 12; ++c) q[c] = r[c];
-    }
+    if (bc_slip_wall(type)) slip_wall_gradients<double>(type, nv, t1, t2, q);
     double fx[4], fy[4], fz[4];
     diff_flux<double>(g, Pb[1], Pb[2], Pb[3], q, q + 4, q + 8, fx, fy, fz);
     if (type == BC_WALL_ADIABATIC) fx[3] = fy[3] = fz[3] = 0.0;
@@ -376,7 +391,8 @@ __global__ __launch_bounds__(256) void k_bc_flux(DevMesh m, const double* __rest
 template <int N, bool GL, bool SPLIT, class RF>
 __global__ __launch_bounds__(Dim<N>::NT) void k_volint(const double* __restrict__ U, DevMesh m,
                                                      const RF* __restrict__ G, const double* __restrict__ Flux,
-                                                     Gas gas, double* __restrict__ Ut, RKStage st) {
+                                                     const double* __restrict__ FluxS, Gas gas,
+                                                     double* __restrict__ Ut, RKStage st) {
   constexpr int n = Dim<N>::n, nf = Dim<N>::nf, np = Dim<N>::np, NT = Dim<N>::NT;
   constexpr int NPRIM = SPLIT ? 6 : 5;   // split: rho,u,v,w,p,H ; weak: staging of f~ (5 comps)
   constexpr int NDIR = SPLIT ? 7 : 0;    // split: Ja^d (3) + viscous contravariant flux (4)
@@ -528,8 +544,9 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_volint(const double* __restrict_
     const double lh = GL ? (is_minus(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N])
                          : (is_minus(ls) ? sB[B.L_HatMinus() + l] : sB[B.L_HatPlus() + l]);
     const RF sl = (RF)(flip == 0 ? lh : -lh);
+    const double* Fs = (flip != 0 && FluxS) ? FluxS : Flux;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) acc[v] = acc[v] + (RF)Flux[5 * fp + v] * sl;
+    for (int v = 0; v < 5; ++v) acc[v] = acc[v] + (RF)Fs[5 * fp + v] * sl;
   }
   const double sj = m.sJ[gid];
   double ut[5];
@@ -698,7 +715,7 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_lift_density(const double* __res
     if (side < m.nBC) {
       double Pm[6], Pb[6], nv[3], t1[3], t2[3];
       const int type = bc_point(m, Um, gas, fp, Pm, Pb, nv, t1, t2);
-      const double rb = type == BC_DIRICHLET ? 0.5 * (Pm[0] + Pb[0]) : (type == BC_WALL_SLIP ? Pm[0] : Pb[0]);
+      const double rb = bc_riemann_type(type) ? 0.5 * (Pm[0] + Pb[0]) : (bc_slip_wall(type) ? Pm[0] : Pb[0]);
       f = (rb - Pm[0]) * se;
     } else {
       f = 0.5 * (Us[5 * (size_t)fp] - Um[5 * (size_t)fp]) * se;
@@ -765,10 +782,10 @@ static hipError_t rhs_n(const DevMesh& m, const DevWork& w, const Gas& gas, int 
     if (nIn > 0) {
       if (split)
         hipLaunchKernelGGL((k_flux<RF, true>), dim3((nIn + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
-                           (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nBCPts, nPts);
+                           (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, w.FluxS, nBCPts, nPts);
       else
         hipLaunchKernelGGL((k_flux<RF, false>), dim3((nIn + 255) / 256), dim3(256), 0, s, m, w.Uf_m, w.Uf_s,
-                           (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, nBCPts, nPts);
+                           (const RF*)w.Gf_m, (const RF*)w.Gf_s, gas, riemann, w.Flux, w.FluxS, nBCPts, nPts);
     }
     if (nBCPts > 0)
       hipLaunchKernelGGL((k_bc_flux<RF>), dim3((nBCPts + 255) / 256), dim3(256), 0, s, m, w.Uf_m, (const RF*)w.Gf_m,
@@ -777,11 +794,11 @@ static hipError_t rhs_n(const DevMesh& m, const DevWork& w, const Gas& gas, int 
   {
     TimedScope ts(tm, K_VOLINT, s);
     if (split)
-      hipLaunchKernelGGL((k_volint<N, GL, true, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut,
-                         st);
+      hipLaunchKernelGGL((k_volint<N, GL, true, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, w.FluxS,
+                         gas, Ut, st);
     else
-      hipLaunchKernelGGL((k_volint<N, GL, false, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux, gas, Ut,
-                         st);
+      hipLaunchKernelGGL((k_volint<N, GL, false, RF>), dim3(K), dim3(NT), 0, s, U, m, (const RF*)w.G, w.Flux,
+                         w.FluxS, gas, Ut, st);
   }
   return hipGetLastError();
 }

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(512) void k_metrics_n(const double* __restrict__ XC
                                                  double* __restrict__ Mg, double* __restrict__ Mh,
                                                  double* __restrict__ sJ, double* __restrict__ NormVec,
                                                  double* __restrict__ TangVec1, double* __restrict__ TangVec2,
-                                                 double* __restrict__ SurfElem) {
+                                                 double* __restrict__ SurfElem, double* __restrict__ SlaveNS) {
   extern __shared__ double sh[];
   const int n = M.n, ng = M.ng, nr = M.nr, np = n * n * n, ngp = ng * ng * ng, nrp = nr * nr * nr, nf = n * n;
   const int N = n - 1;
@@ -239,8 +239,8 @@ __global__ __launch_bounds__(512) void k_metrics_n(const double* __restrict__ XC
     const int ls = x / nf + 1, pq = x - (ls - 1) * nf;
     const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
     const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
-    if (flip != 0) continue;
-    const int* ab = m.s2v2 + 2 * (pq + nf * (0 + 5 * (ls - 1)));
+    if (flip != 0 && !SlaveNS) continue;
+    const int* ab = m.s2v2 + 2 * (pq + nf * (flip + 5 * (ls - 1)));
     const int a0 = ab[0], b0 = ab[1];
     const int l = (ls == 5 || ls == 2 || ls == 1) ? 0 : N;
     const int dir = (ls == 5 || ls == 3) ? 0 : (ls == 2 || ls == 4) ? 1 : 2;
@@ -253,6 +253,12 @@ __global__ __launch_bounds__(512) void k_metrics_n(const double* __restrict__ XC
       }
     const int nd = NormalDirs[ls] - 1, td = TangDirs[ls] - 1;
     const double ns = NormalSigns[ls];
+    if (flip != 0) {
+      // the slave element's own n*SurfElem of the side point, oriented as the master's normal
+      const size_t fs = (size_t)pq + (size_t)nf * side;
+      for (int c = 0; c < 3; ++c) SlaveNS[3 * fs + c] = -ns * jf[nd + 3 * c];
+      continue;
+    }
     const double se = sqrt(jf[nd] * jf[nd] + jf[nd + 3] * jf[nd + 3] + jf[nd + 6] * jf[nd + 6]);
     double nv[3], t1[3];
     for (int c = 0; c < 3; ++c) nv[c] = ns * jf[nd + 3 * c] / se;
@@ -276,7 +282,8 @@ __global__ __launch_bounds__(512) void k_metrics_n(const double* __restrict__ XC
 
 int compute_metrics_gpu(int N, int nodeType, int NGeo, int nElems, const double* NodeCoords, const DevMesh& m,
                         double* Mf, double* Mg, double* Mh, double* sJ, double* xGP, double* NormVec,
-                        double* TangVec1, double* TangVec2, double* SurfElem, hipStream_t s, std::string& err) {
+                        double* TangVec1, double* TangVec2, double* SurfElem, double* SlaveNS, hipStream_t s,
+                        std::string& err) {
   using namespace basis;
   MetricMats M(N, NGeo);
   const int n = N + 1, ng = NGeo + 1, NGeoRef = 3 * NGeo;
@@ -337,7 +344,7 @@ int compute_metrics_gpu(int N, int nodeType, int NGeo, int nElems, const double*
   if (e == hipSuccess) {
     // k_metrics_n keeps up to 4 nodes per lane in registers: np <= 4*512
     hipLaunchKernelGGL(k_metrics_n, dim3(nElems), dim3(512), shN, s, XCLN, dXCLG, detRef, dm, M, NGeo, m, Mf, Mg, Mh,
-                       sJ, NormVec, TangVec1, TangVec2, SurfElem);
+                       sJ, NormVec, TangVec1, TangVec2, SurfElem, SlaveNS);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);

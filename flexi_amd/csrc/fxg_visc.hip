@@ -91,12 +91,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : (N == 7 
 
 // One lane per boundary side point, after k_visc_faces: the outer slot's viscous flux, so that with
 // the inner element's half it makes the boundary's viscous flux (getboundaryflux.f90:561-776):
-// Dirichlet 0.5 (f_in + f_b).n, walls f_wall.n with the wall's velocity (none, or the tangential
-// part at a slip wall) and the inner gradients (without their normal part at a slip wall; no heat
-// flux at an adiabatic wall). n SurfElem is the side's.
+// Riemann type 0.5 (f_in + f_b).n with the boundary state's velocity, walls f_wall.n with the
+// wall's velocity (none, or the tangential part at a slip wall) and the inner gradients (slip walls:
+// as slip_wall_gradients leaves them; no heat flux at an adiabatic wall). n SurfElem is the side's.
 template <class RF>
 __global__ __launch_bounds__(256) void k_bc_visc(DevMesh m, const double* __restrict__ FU, const RF* __restrict__ gBC,
-                                               RF* __restrict__ FG, Gas gas, int nf, int nPts) {
+                                               RF* __restrict__ FG, const double* __restrict__ bcP, Gas gas, int nf,
+                                               int nPts) {
   const int fp = blockIdx.x * blockDim.x + threadIdx.x;
   if (fp >= nPts) return;
   const int side = fp / nf, pq = fp - side * nf;
@@ -109,19 +110,21 @@ __global__ __launch_bounds__(256) void k_bc_visc(DevMesh m, const double* __rest
 #pragma unroll
   for (int c = 0; c < 12; ++c) q[c] = gBC[12 * (size_t)fp + c];
   RF ub[3] = {RF(0), RF(0), RF(0)};
+  const bool riemann = bc_riemann_type(type);
   if (type == BC_DIRICHLET) {
     const double* ref = m.refPrim + 6 * m.bcInfo[2 * side + 1];
     ub[0] = (RF)ref[1]; ub[1] = (RF)ref[2]; ub[2] = (RF)ref[3];
-  } else if (type == BC_WALL_SLIP) {
+  } else if (riemann) {  // the boundary state's velocity, as k_bc_ghost formed it
+    ub[0] = (RF)bcP[4 * (size_t)fp + 1]; ub[1] = (RF)bcP[4 * (size_t)fp + 2]; ub[2] = (RF)bcP[4 * (size_t)fp + 3];
+  } else if (bc_slip_wall(type)) {
     const Prim p = cons_to_prim(FU + 5 * in, gas);
     const RF vn = (RF)p.u * nh[0] + (RF)p.v * nh[1] + (RF)p.w * nh[2];
     ub[0] = (RF)p.u - vn * nh[0]; ub[1] = (RF)p.v - vn * nh[1]; ub[2] = (RF)p.w - vn * nh[2];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const RF gn = nh[0] * q[v] + nh[1] * q[4 + v] + nh[2] * q[8 + v];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) q[4 * a + v] = q[4 * a + v] - nh[a] * gn;
-    }
+    const RF t1[3] = {(RF)m.TangVec1[3 * (size_t)fp], (RF)m.TangVec1[3 * (size_t)fp + 1],
+                      (RF)m.TangVec1[3 * (size_t)fp + 2]};
+    const RF t2[3] = {(RF)m.TangVec2[3 * (size_t)fp], (RF)m.TangVec2[3 * (size_t)fp + 1],
+                      (RF)m.TangVec2[3 * (size_t)fp + 2]};
+    slip_wall_gradients<RF>(type, nh, t1, t2, q);
   }
   RF bx[4], by[4], bz[4];
   diff_flux<RF>(gt, ub[0], ub[1], ub[2], q, q + 4, q + 8, bx, by, bz);
@@ -131,16 +134,16 @@ __global__ __launch_bounds__(256) void k_bc_visc(DevMesh m, const double* __rest
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const RF fb = (bx[c] * nh[0] + by[c] * nh[1] + bz[c] * nh[2]) * se;
-    fo[c] = type == BC_DIRICHLET ? -fb : fi[c] - RF(2) * fb;
+    fo[c] = riemann ? -fb : fi[c] - RF(2) * fb;
   }
 }
 
 // One lane per boundary side point: the outer slot of the boundary side, from the inner face
-// state and GetBoundaryState (getboundaryflux.f90:224-472). Dirichlet: the boundary state itself
-// (the Riemann solver and the lifting then see it as FLEXI's boundary flux does). Walls: a state
-// whose lifting variables are 2 f - q_in, f the boundary lifting value (Lifting_GetBoundaryFlux,
-// :855-939), so that the BR1 difference (q_out - q_in)/2 is FLEXI's f - q_in; their inviscid flux
-// (0, p_wall n, 0) is formed by the stage kernel from bcP.
+// state and GetBoundaryState (getboundaryflux.f90:224-472). Riemann type: the boundary state itself
+// (the Riemann solver and the lifting then see it as FLEXI's boundary flux does); its velocity also
+// goes to bcP for k_bc_visc. Walls: a state whose lifting variables are 2 f - q_in, f the boundary
+// lifting value (Lifting_GetBoundaryFlux, :855-939), so that the BR1 difference (q_out - q_in)/2 is
+// FLEXI's f - q_in; their inviscid flux (0, p_wall n, 0) is formed by the stage kernel from bcP.
 __global__ __launch_bounds__(256) void k_bc_ghost(DevMesh m, double* __restrict__ FU, double* __restrict__ bcP,
                                                 Gas gas, int nf, int nPts) {
   const int fp = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,11 +161,14 @@ __global__ __launch_bounds__(256) void k_bc_ghost(DevMesh m, double* __restrict_
   }
   const int type = m.bcInfo[2 * side], st = m.bcInfo[2 * side + 1];
   double Pb[6];
-  boundary_state(gas, type, st >= 0 ? m.refPrim + 6 * st : nullptr, Pm, nv, t1, t2, Pb);
+  boundary_state(gas, type, st >= 0 ? m.refPrim + 6 * st : nullptr, m.bcData ? m.bcData + 6 * (size_t)fp : nullptr,
+                 Pm, nv, t1, t2, Pb);
   double G[6];
-  if (type == BC_DIRICHLET) {
+  double* bp = bcP + 4 * (size_t)fp;
+  if (bc_riemann_type(type)) {
 #pragma unroll
     for (int v = 0; v < 6; ++v) G[v] = Pb[v];
+    bp[0] = Pb[4]; bp[1] = Pb[1]; bp[2] = Pb[2]; bp[3] = Pb[3];
   } else {
     double F[4];
     boundary_lifting_flux(type, Pm, Pb, F);
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(256) void k_bc_ghost(DevMesh m, double* __restrict_
     G[1] = Pm[1] + 2.0 * F[0]; G[2] = Pm[2] + 2.0 * F[1]; G[3] = Pm[3] + 2.0 * F[2];
     G[5] = Pm[5] + 2.0 * F[3];
     G[4] = G[0] * gas.R * G[5];
-    bcP[fp] = Pb[4];
+    bp[0] = Pb[4];
   }
   double* o = FU + 5 * out;
   o[0] = G[0];
@@ -188,7 +194,7 @@ hipError_t launch_bc_ghost(const DevMesh& m, double* FU, double* bcP, const Gas&
 }
 
 hipError_t launch_visc_faces(const DevMesh& m, bool fp32, const double* U, const double* Jsoa, const double* FU,
-                             void* FG, const Gas& gas, void* gBC, hipStream_t s) {
+                             void* FG, const Gas& gas, void* gBC, const double* bcP, hipStream_t s) {
   const bool bc = m.nBC > 0;
   switch (m.N) {
 #define FXG_LAUNCH(NN, RF, BC)                                                                                    \
@@ -211,10 +217,10 @@ hipError_t launch_visc_faces(const DevMesh& m, bool fp32, const double* U, const
     const int nf = (m.N + 1) * (m.N + 1), nPts = m.nBC * nf;
     if (fp32)
       hipLaunchKernelGGL((k_bc_visc<float>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, FU, (const float*)gBC,
-                         (float*)FG, gas, nf, nPts);
+                         (float*)FG, bcP, gas, nf, nPts);
     else
       hipLaunchKernelGGL((k_bc_visc<double>), dim3((nPts + 255) / 256), dim3(256), 0, s, m, FU, (const double*)gBC,
-                         (double*)FG, gas, nf, nPts);
+                         (double*)FG, bcP, gas, nf, nPts);
   }
   return hipGetLastError();
 }

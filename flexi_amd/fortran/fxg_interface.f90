@@ -33,6 +33,7 @@ TYPE,BIND(C),PUBLIC :: fxg_mesh
 END TYPE fxg_mesh
 
 PUBLIC :: fxg_last_error, fxg_version, fxg_create, fxg_create_from_nodes, fxg_destroy, fxg_set_boundaries
+PUBLIC :: fxg_set_boundary_data
 PUBLIC :: fxg_set_state, fxg_get_state, fxg_get_ut, fxg_get_gradients, fxg_get_metrics
 PUBLIC :: fxg_tgv_analyze
 PUBLIC :: fxg_dg_time_derivative, fxg_timestep_lserkw2, fxg_calc_timestep
@@ -75,6 +76,12 @@ INTERFACE
     INTEGER(C_INT),VALUE      :: nBCs, nRefState
     INTEGER(C_INT),INTENT(IN) :: BC(*), BoundaryType(*)
     REAL(C_DOUBLE),INTENT(IN) :: RefStatePrim(*)
+  END FUNCTION
+  ! BCDataPrim(6,0:N,0:N,nBCSides) of the boundaries of types 12 and 121 (equation_vars.f90)
+  INTEGER(C_INT) FUNCTION fxg_set_boundary_data(ctx,BCDataPrim) BIND(C,NAME='fxg_set_boundary_data')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR),VALUE         :: ctx
+    REAL(C_DOUBLE),INTENT(IN) :: BCDataPrim(*)
   END FUNCTION
   INTEGER(C_INT) FUNCTION fxg_set_state(ctx,U) BIND(C,NAME='fxg_set_state')
     IMPORT :: C_INT, C_DOUBLE, C_PTR

@@ -163,6 +163,14 @@ def run(case: Case, out_dir: str, precision: str = "fp64", restart: str | None =
         bt = mesh.BoundaryType.copy(order="F")
         bt[(bt[:, 0] == 2) & (bt[:, 1] == 0), 1] = ref_prim.shape[1]
         mesh.BoundaryType = bt
+    if mesh.nBCSides and ref_prim is not None:
+        # subsonic inflow (27): InitBC turns the RefState's (alpha, beta) in degrees into the inflow
+        # direction vector a(1:3) in place (getboundaryflux.f90:176-186)
+        ref_prim = np.asfortranarray(ref_prim.copy())
+        for st in sorted({int(x) for x in mesh.BoundaryType[mesh.BoundaryType[:, 0] == 27, 1]}):
+            ta, tb = np.tan(np.pi / 180.0 * ref_prim[1, st - 1]), np.tan(np.pi / 180.0 * ref_prim[2, st - 1])
+            nrm = np.sqrt(1.0 + ta ** 2 + tb ** 2)
+            ref_prim[1:4, st - 1] = (1.0 / nrm, ta / nrm, tb / nrm)
     s = DGSolver(mesh, case.N, node_type=case.node_type, split_dg=case.split, riemann=case.riemann,
                  kappa=case.kappa, R=case.R, Pr=case.Pr, mu0=case.mu0, precision=precision, scheme=case.scheme,
                  CFLScale=case.CFL, DFLScale=case.DFL, ref_state_prim=ref_prim)

@@ -41,7 +41,7 @@ class DGSolver:
     def __init__(self, mesh: Mesh, N: int, node_type="gauss-lobatto", split_dg="pi", riemann="roe",
                  kappa=1.4, R=71.42857, Pr=0.72, mu0=6.25e-4, precision="fp64", metrics: Optional[dict] = None,
                  scheme: LSERK = CARPENTER_RK4_5, CFLScale=0.9, DFLScale=0.9,
-                 ref_state_prim: Optional[np.ndarray] = None):
+                 ref_state_prim: Optional[np.ndarray] = None, bc_data: Optional[np.ndarray] = None):
         L = _lib.lib()
         self.mesh = mesh
         self.N = N
@@ -73,6 +73,10 @@ class DGSolver:
                         _f(ref_state_prim if ref_state_prim is not None else np.zeros((6, 1))))
             bc, bt, rp = self._bc
             rc = L.fxg_set_boundaries(self.ctx, bt.shape[0], _ptr(bc), _ptr(bt), rp.shape[1], _ptr(rp))
+            if rc == 0 and bc_data is not None:
+                # BCDataPrim(6,0:N,0:N,nBCSides) of the boundaries of types 12 and 121
+                self._bc_data = _f(bc_data)
+                rc = L.fxg_set_boundary_data(self.ctx, _ptr(self._bc_data))
             if rc != 0:
                 msg = L.fxg_last_error().decode()
                 L.fxg_destroy(self.ctx)

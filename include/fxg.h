@@ -100,13 +100,26 @@ int fxg_destroy(fxg_ctx* ctx);
 /* Boundary conditions of the boundary sides (InitBC, src/equations/navierstokes/idealgas/
  * getboundaryflux.f90:64-218, with FLEXI's arrays): BC(nBCSides) the boundary index of each side
  * (mesh_vars.f90), BoundaryType(nBCs,3) = (type, state, alpha) in Fortran layout, RefStatePrim(6,
- * nRefState) = (rho,u,v,w,p,T) per RefState (equation.f90:126-141). Types: 2 Dirichlet with
- * state = RefState index (>= 1), 3 adiabatic wall, 4 isothermal wall (temperature of RefState
- * `state`), 9 slip wall (getboundaryflux.f90:259-465, 480-789, 855-939). Both stage pipelines
- * support them; the discrete adjoint does not. Call it before the first right-hand side, with no
- * work of the context in flight. No-op for a mesh without boundary sides. */
+ * nRefState) = (rho,u,v,w,p,T) per RefState (equation.f90:126-141) as InitBC leaves it (for type 27
+ * the entries 2..4 hold the inflow direction vector). Types (getboundaryflux.f90:224-472,
+ * 480-789, 855-939): 2 Dirichlet with state = RefState index (>= 1); 12 and 121 Dirichlet with a
+ * per-point state given by fxg_set_boundary_data; 23 outflow at the Mach number RefStatePrim(2,
+ * state); 24 pressure outflow and 25 subsonic outflow at RefStatePrim(5,state); 27 subsonic
+ * inflow (total temperature, direction, total pressure of RefState `state`); 3 adiabatic wall; 4
+ * isothermal wall (temperature of RefState `state`); 9 slip wall; 91 slip wall with the wall's
+ * velocity-gradient treatment. Type 22 (an exact function of time) is not evaluated by the
+ * library. Both stage pipelines support them; the discrete adjoint does not. Call it before the
+ * first right-hand side, with no work of the context in flight. No-op for a mesh without
+ * boundary sides. */
 int fxg_set_boundaries(fxg_ctx* ctx, int nBCs, const int* BC, const int* BoundaryType, int nRefState,
                        const double* RefStatePrim);
+/* The boundary states of the sides of types 12 and 121, BCDataPrim(6,0:N,0:N,nBCSides) =
+ * (rho,u,v,w,p,T) per boundary side point in FLEXI's layout (equation_vars.f90; filled there by
+ * ReadBCFlow from a State file for 12, by ExactFunc at the start time for 121, getboundaryflux.f90:
+ * 64-218, 946-1040). Replaces those two fills; entries of sides of other types are not read but
+ * must be physical. Call it after fxg_set_boundaries and before a right-hand side, with no work of
+ * the context in flight. */
+int fxg_set_boundary_data(fxg_ctx* ctx, const double* BCDataPrim);
 
 /* Host <-> device state transfer in FLEXI layout U(5,0:N,0:N,0:N,nElems). */
 int fxg_set_state(fxg_ctx* ctx, const double* U);

@@ -286,6 +286,7 @@ struct or_ctx {
   int nBCSides = 0, nBCs = 0, nRefState = 0;
   std::vector<int> BC, BoundaryType;  // BC(nBCSides), BoundaryType(nBCs,3): type, state, alpha
   std::vector<double> RefStatePrim;   // (6,nRefState)
+  std::vector<double> BCDataPrim;     // (6,0:N,0:N,nBCSides) for types 12/121, or empty
 };
 
 namespace {
@@ -306,33 +307,104 @@ double pressure_riemann(const EosC& e, const double* P) {
   return P[PRES] + P[VEL1] / ar * 0.5 * (P[VEL1] + std::sqrt(P[VEL1] * P[VEL1] + 4. * ar * (P[PRES] + br)));
 }
 
-// GetBoundaryState, idealgas/getboundaryflux.f90:224-472, for BC types 2 (RefState, BCState >= 1),
-// 3 (adiabatic wall), 4 (isothermal wall) and 9 (slip wall); ref = RefStatePrim(:,BCState).
-void boundary_state(const EosC& e, int type, const double* ref, const double* Pm, const double* nv,
+inline bool riemann_type_bc(int type) {
+  return type == 2 || type == 12 || type == 121 || type == 23 || type == 24 || type == 25 || type == 27;
+}
+
+// GetBoundaryState, idealgas/getboundaryflux.f90:224-472: types 2 (RefState, BCState >= 1), 12/121
+// (BCDataPrim of the point, data), 3 (adiabatic wall), 4 (isothermal wall), 9/91 (slip wall), 23
+// (outflow Mach number), 24 (pressure outflow), 25 (subsonic outflow), 27 (subsonic inflow); ref =
+// RefStatePrim(:,BCState), as InitBC leaves it (type 27: Tt, direction a(1:3), pt).
+void boundary_state(const EosC& e, int type, const double* ref, const double* data, const double* Pm, const double* nv,
                     const double* t1, const double* t2, double* Pb) {
   if (type == 2) {
     for (int v = 0; v < 6; ++v) Pb[v] = ref[v];
     return;
   }
+  if (type == 12 || type == 121) {
+    for (int v = 0; v < 6; ++v) Pb[v] = data[v];
+    return;
+  }
+  // transform the inner state into the normal system (:302-312)
   Pb[DENS] = Pm[DENS];
   Pb[VEL1] = Pm[VEL1] * nv[0] + Pm[VEL2] * nv[1] + Pm[VEL3] * nv[2];
   Pb[VEL2] = Pm[VEL1] * t1[0] + Pm[VEL2] * t1[1] + Pm[VEL3] * t1[2];
   Pb[VEL3] = Pm[VEL1] * t2[0] + Pm[VEL2] * t2[1] + Pm[VEL3] * t2[2];
   Pb[PRES] = Pm[PRES];
   Pb[TEMP] = Pm[TEMP];
-  if (type == 3 || type == 4) {
+  const double kappa = e.kappa, kappaM1 = e.kappaM1, sKappaM1 = e.sKappaM1, R = e.R;
+  if (type == 3 || type == 4) {  // :314-336
     Pb[PRES] = pressure_riemann(e, Pb);
     Pb[VEL1] = Pb[VEL2] = Pb[VEL3] = 0.;
     Pb[TEMP] = type == 3 ? Pm[TEMP] : ref[TEMP];
-    Pb[DENS] = Pb[PRES] / (Pb[TEMP] * e.R);
-  } else {  // 9
+    Pb[DENS] = Pb[PRES] / (Pb[TEMP] * R);
+  } else if (type == 9 || type == 91) {  // :338-350
     Pb[PRES] = pressure_riemann(e, Pb);
     Pb[VEL1] = 0.;
     Pb[DENS] = Pm[DENS];
-    Pb[TEMP] = Pb[PRES] / (Pb[DENS] * e.R);
+    Pb[TEMP] = Pb[PRES] / (Pb[DENS] * R);
+  } else if (type == 23) {  // :358-383
+    const double MaOut = ref[1];
+    const double c = std::sqrt(kappa * Pb[PRES] / Pb[DENS]);
+    const double Ma = Pb[VEL1] / c;
+    double pb;
+    if (Ma < 1) {
+      const double pt = Pb[PRES] * std::pow(1 + 0.5 * (kappa - 1) * Ma * Ma, kappa * sKappaM1);
+      pb = pt * std::pow(1 + 0.5 * (kappa - 1) * MaOut * MaOut, -kappa * sKappaM1);
+    } else {
+      pb = Pb[PRES] + 0.5 * Pb[DENS] * (Pb[VEL1] * Pb[VEL1] + Pb[VEL2] * Pb[VEL2] + Pb[VEL3] * Pb[VEL3]);
+    }
+    Pb[DENS] = kappa * pb / (c * c);
+    Pb[PRES] = pb;
+    Pb[TEMP] = Pb[PRES] / (R * Pb[DENS]);
+  } else if (type == 24) {  // :385-403
+    const double c = std::sqrt(kappa * Pb[PRES] / Pb[DENS]);
+    const double Ma = Pb[VEL1] / c;
+    if (Ma < 1) {
+      const double pb = ref[PRES];
+      Pb[DENS] = kappa * pb / (c * c);
+      Pb[PRES] = pb;
+      Pb[TEMP] = Pb[PRES] / (R * Pb[DENS]);
+    }
+  } else if (type == 25) {  // :405-431
+    const double c = std::sqrt(kappa * Pb[PRES] / Pb[DENS]);
+    const double Ma = Pb[VEL1] / c;
+    double pb;
+    if (Ma < 1) pb = ref[PRES];
+    else pb = Pb[PRES] + 0.5 * Pb[DENS] * (Pb[VEL1] * Pb[VEL1] + Pb[VEL2] * Pb[VEL2] + Pb[VEL3] * Pb[VEL3]);
+    if (Pb[VEL1] < 0.) {
+      Pb[VEL1] = std::fabs(Pb[VEL1]);
+      Pb[VEL2] = 0.;
+      Pb[VEL3] = 0.;
+    }
+    Pb[DENS] = kappa * pb / (c * c);
+    Pb[PRES] = ref[PRES];
+    Pb[TEMP] = Pb[PRES] / (R * Pb[DENS]);
+  } else {  // 27, :433-472
+    const double Tt = ref[0], pt = ref[4];
+    const double* a = ref + 1;
+    const double A = -1. * (a[0] * nv[0] + a[1] * nv[1] + a[2] * nv[2]);
+    const double c = std::sqrt(kappa * Pb[PRES] / Pb[DENS]);
+    const double Rplus = -Pb[VEL1] - 2. * c / kappaM1;
+    const double tmp1 = A * A + 2. / kappaM1;
+    const double tmp2 = 2 * Rplus;
+    const double tmp3 = kappaM1 / 2. * Rplus * Rplus - kappa * R * Tt * A * A;
+    const double cb = std::fmax((-tmp2 + std::sqrt(tmp2 * tmp2 - 4 * tmp1 * tmp3)) / (2 * tmp1),
+                                (-tmp2 - std::sqrt(tmp2 * tmp2 - 4 * tmp1 * tmp3)) / (2 * tmp1));
+    const double Tb = cb * cb / (kappa * R);
+    const double Ma = std::sqrt(2. / kappaM1 * (Tt / Tb - 1.));
+    const double pb = pt * std::pow(1. + 0.5 * kappaM1 * Ma * Ma, -kappa / kappaM1);
+    const double U = Ma * std::sqrt(kappa * R * Tb);
+    Pb[DENS] = pb / (R * Tb);
+    Pb[VEL1] = U * (a[0] * nv[0] + a[1] * nv[1] + a[2] * nv[2]);
+    Pb[VEL2] = U * (a[0] * t1[0] + a[1] * t1[1] + a[2] * t1[2]);
+    Pb[VEL3] = U * (a[0] * t2[0] + a[1] * t2[1] + a[2] * t2[2]);
+    Pb[PRES] = pb;
+    Pb[TEMP] = Tb;
   }
-  const double a = Pb[VEL1], b = Pb[VEL2], c = Pb[VEL3];
-  for (int d = 0; d < 3; ++d) Pb[VEL1 + d] = a * nv[d] + b * t1[d] + c * t2[d];
+  // rotate the state back to the physical system (:476-481)
+  const double va = Pb[VEL1], vb = Pb[VEL2], vc = Pb[VEL3];
+  for (int d = 0; d < 3; ++d) Pb[VEL1 + d] = va * nv[d] + vb * t1[d] + vc * t2[d];
 }
 
 // Lifting_GetBoundaryFlux, idealgas/getboundaryflux.f90:855-939 (strong form, lifting variables
@@ -341,10 +413,64 @@ void lifting_boundary_flux(int type, const double* Pm, const double* Pb, double*
   const int PL[4] = {VEL1, VEL2, VEL3, TEMP};
   for (int v = 0; v < 4; ++v) {
     double f;
-    if (type == 2) f = 0.5 * (Pm[PL[v]] + Pb[PL[v]]);
+    if (riemann_type_bc(type)) f = 0.5 * (Pm[PL[v]] + Pb[PL[v]]);
     else if (type == 3 || type == 4) f = v < 3 ? 0. : Pb[TEMP];
-    else f = v < 3 ? Pb[PL[v]] : Pm[TEMP];
+    else f = v < 3 ? Pb[PL[v]] : Pm[TEMP];  // 9, 91
     F[v] = f - Pm[PL[v]];
+  }
+}
+
+// The gradients the viscous flux of a slip wall sees, getboundaryflux.f90:622-776 (lifting
+// variables u,v,w,T in g*[0..3]): 9 projects every gradient on the wall, (I - n n^T) grad; 91 does
+// so for T, and for the velocity removes in the normal/tangential frame the normal derivatives of
+// the tangential components and the tangential derivatives of the normal component.
+void slip_gradients(int type, const double* nv, const double* tv1, const double* tv2, const double* gx,
+                    const double* gy, const double* gz, double* qx, double* qy, double* qz) {
+  double M[3][3];
+  M[0][0] = 1. - nv[0] * nv[0];
+  M[1][1] = 1. - nv[1] * nv[1];
+  M[2][2] = 1. - nv[2] * nv[2];
+  M[0][1] = -nv[0] * nv[1];
+  M[0][2] = -nv[0] * nv[2];
+  M[2][1] = -nv[2] * nv[1];
+  M[1][0] = M[0][1];
+  M[2][0] = M[0][2];
+  M[1][2] = M[2][1];
+  if (type == 9) {
+    for (int v = 0; v < 4; ++v) {
+      qx[v] = M[0][0] * gx[v] + M[0][1] * gy[v] + M[0][2] * gz[v];
+      qy[v] = M[1][0] * gx[v] + M[1][1] * gy[v] + M[1][2] * gz[v];
+      qz[v] = M[2][0] * gx[v] + M[2][1] * gy[v] + M[2][2] * gz[v];
+    }
+    return;
+  }
+  const int T = 3;
+  qx[T] = M[0][0] * gx[T] + M[0][1] * gy[T] + M[0][2] * gz[T];
+  qy[T] = M[1][0] * gx[T] + M[1][1] * gy[T] + M[1][2] * gz[T];
+  qz[T] = M[2][0] * gx[T] + M[2][1] * gy[T] + M[2][2] * gz[T];
+  auto dot = [](const double* a, const double* g) { return a[0] * g[0] + a[1] * g[1] + a[2] * g[2]; };
+  double gx_vN = dot(nv, gx), gx_vT1 = dot(tv1, gx), gx_vT2 = dot(tv2, gx);
+  double gy_vN = dot(nv, gy), gy_vT1 = dot(tv1, gy), gy_vT2 = dot(tv2, gy);
+  double gz_vN = dot(nv, gz), gz_vT1 = dot(tv1, gz), gz_vT2 = dot(tv2, gz);
+  const double gn_vN = nv[0] * gx_vN + nv[1] * gy_vN + nv[2] * gz_vN;
+  const double gn_vT1 = 0., gn_vT2 = 0., gt1_vN = 0., gt2_vN = 0.;
+  const double gt1_vT1 = tv1[0] * gx_vT1 + tv1[1] * gy_vT1 + tv1[2] * gz_vT1;
+  const double gt1_vT2 = tv1[0] * gx_vT2 + tv1[1] * gy_vT2 + tv1[2] * gz_vT2;
+  const double gt2_vT1 = tv2[0] * gx_vT1 + tv2[1] * gy_vT1 + tv2[2] * gz_vT1;
+  const double gt2_vT2 = tv2[0] * gx_vT2 + tv2[1] * gy_vT2 + tv2[2] * gz_vT2;
+  gx_vN = nv[0] * gn_vN + tv1[0] * gt1_vN + tv2[0] * gt2_vN;
+  gx_vT1 = nv[0] * gn_vT1 + tv1[0] * gt1_vT1 + tv2[0] * gt2_vT1;
+  gx_vT2 = nv[0] * gn_vT2 + tv1[0] * gt1_vT2 + tv2[0] * gt2_vT2;
+  gy_vN = nv[1] * gn_vN + tv1[1] * gt1_vN + tv2[1] * gt2_vN;
+  gy_vT1 = nv[1] * gn_vT1 + tv1[1] * gt1_vT1 + tv2[1] * gt2_vT1;
+  gy_vT2 = nv[1] * gn_vT2 + tv1[1] * gt1_vT2 + tv2[1] * gt2_vT2;
+  gz_vN = nv[2] * gn_vN + tv1[2] * gt1_vN + tv2[2] * gt2_vN;
+  gz_vT1 = nv[2] * gn_vT1 + tv1[2] * gt1_vT1 + tv2[2] * gt2_vT1;
+  gz_vT2 = nv[2] * gn_vT2 + tv1[2] * gt1_vT2 + tv2[2] * gt2_vT2;
+  for (int a = 0; a < 3; ++a) {
+    qx[a] = nv[a] * gx_vN + tv1[a] * gx_vT1 + tv2[a] * gx_vT2;
+    qy[a] = nv[a] * gy_vN + tv1[a] * gy_vT1 + tv2[a] * gy_vT2;
+    qz[a] = nv[a] * gz_vN + tv1[a] * gz_vT1 + tv2[a] * gz_vT2;
   }
 }
 
@@ -353,7 +479,7 @@ void boundary_flux(const or_ctx* c, int type, const double* Pm, const double* Pb
                    const double* gy, const double* gz, const double* nv, const double* t1, const double* t2,
                    double* F) {
   const EosC& e = c->e;
-  if (type == 2) {  // Riemann-type: Riemann solver against the boundary state + ViscousFlux
+  if (riemann_type_bc(type)) {  // Riemann-type: Riemann solver against the boundary state + ViscousFlux
     double Um[5], Ub[5];
     prim_to_cons(e, Pm, Um);
     prim_to_cons(e, Pb, Ub);
@@ -373,16 +499,9 @@ void boundary_flux(const or_ctx* c, int type, const double* Pm, const double* Pb
   if (type == 3 || type == 4) {
     eval_diff_flux(e, Pb, gx, gy, gz, fd, gd, hd);
     if (type == 3) fd[ENER] = gd[ENER] = hd[ENER] = 0.;
-  } else {  // 9: gradients without their normal component, (I - n n^T) grad
-    double M[3][3];
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) M[a][b] = a == b ? 1. - nv[a] * nv[a] : -nv[a] * nv[b];
+  } else {  // 9, 91
     double qx[4], qy[4], qz[4];
-    for (int v = 0; v < 4; ++v) {
-      qx[v] = M[0][0] * gx[v] + M[0][1] * gy[v] + M[0][2] * gz[v];
-      qy[v] = M[1][0] * gx[v] + M[1][1] * gy[v] + M[1][2] * gz[v];
-      qz[v] = M[2][0] * gx[v] + M[2][1] * gy[v] + M[2][2] * gz[v];
-    }
+    slip_gradients(type, nv, t1, t2, gx, gy, gz, qx, qy, qz);
     eval_diff_flux(e, Pb, qx, qy, qz, fd, gd, hd);
   }
   for (int v = 1; v < 5; ++v) F[v] = F[v] + nv[0] * fd[v] + nv[1] * gd[v] + nv[2] * hd[v];
@@ -390,8 +509,13 @@ void boundary_flux(const or_ctx* c, int type, const double* Pm, const double* Pb
 
 const double* bc_ref(const or_ctx* c, int s) {
   const int iBC = c->BC[s] - 1;
-  const int state = c->BoundaryType[iBC + c->nBCs];
+  const int type = c->BoundaryType[iBC], state = c->BoundaryType[iBC + c->nBCs];
+  if (type == 12 || type == 121 || type == 3 || type == 9 || type == 91) return nullptr;
   return state >= 1 ? &c->RefStatePrim[6 * (state - 1)] : nullptr;
+}
+// BCDataPrim(:,p,q,SideID) of boundary side point fi (types 12/121), or null
+const double* bc_data(const or_ctx* c, size_t fi) {
+  return c->BCDataPrim.empty() ? nullptr : &c->BCDataPrim[6 * fi];
 }
 inline int bc_type(const or_ctx* c, int s) { return c->BoundaryType[c->BC[s] - 1]; }
 
@@ -544,7 +668,7 @@ void lifting_br1(or_ctx* c) {
     for (int pq = 0; pq < nf; ++pq) {
       const size_t fi = (size_t)nf * s + pq;
       double Pb[6], F[4];
-      boundary_state(c->e, type, ref, &c->UPrim_master[6 * fi], &c->NormVec[3 * fi], &c->TangVec1[3 * fi],
+      boundary_state(c->e, type, ref, bc_data(c, fi), &c->UPrim_master[6 * fi], &c->NormVec[3 * fi], &c->TangVec1[3 * fi],
                      &c->TangVec2[3 * fi], Pb);
       lifting_boundary_flux(type, &c->UPrim_master[6 * fi], Pb, F);
       for (int v = 0; v < 4; ++v) {
@@ -706,7 +830,7 @@ void fill_flux(or_ctx* c) {
       const double *Pm = &c->UPrim_master[6 * fi], *nv = &c->NormVec[3 * fi], *t1 = &c->TangVec1[3 * fi],
                    *t2 = &c->TangVec2[3 * fi];
       double Pb[6], F[5];
-      boundary_state(c->e, type, ref, Pm, nv, t1, t2, Pb);
+      boundary_state(c->e, type, ref, bc_data(c, fi), Pm, nv, t1, t2, Pb);
       boundary_flux(c, type, Pm, Pb, &c->gx_m[4 * fi], &c->gy_m[4 * fi], &c->gz_m[4 * fi], nv, t1, t2, F);
       for (int v = 0; v < 5; ++v) c->Flux_master[v + 5 * fi] = F[v] * c->SurfElem[fi];
     }
@@ -758,6 +882,20 @@ void or_split_volume_flux_pi(const double* URef, const double* UPrimRef, const d
 void or_eval_diff_flux_point(const or_params* p, const double* UPrim, const double* gx, const double* gy,
                              const double* gz, double* f, double* g, double* h) {
   eval_diff_flux(make_eos(p), UPrim, gx, gy, gz, f, g, h);
+}
+void or_boundary_point(const or_params* p, int type, const double* ref, const double* data, const double* Pm,
+                       const double* nv, const double* t1, const double* t2, const double* gx, const double* gy,
+                       const double* gz, double* Pb, double* Flift, double* Flux) {
+  or_ctx c;
+  c.p = *p;
+  c.e = make_eos(p);
+  boundary_state(c.e, type, ref, data, Pm, nv, t1, t2, Pb);
+  lifting_boundary_flux(type, Pm, Pb, Flift);
+  boundary_flux(&c, type, Pm, Pb, gx, gy, gz, nv, t1, t2, Flux);
+}
+void or_slip_gradients(int type, const double* nv, const double* t1, const double* t2, const double* gx,
+                       const double* gy, const double* gz, double* qx, double* qy, double* qz) {
+  slip_gradients(type, nv, t1, t2, gx, gy, gz, qx, qy, qz);
 }
 
 or_ctx* or_init(const or_params* p, int nElems, int nSides, const int* SideToElem, const int* ElemToSide,
@@ -823,8 +961,9 @@ int or_set_boundaries(or_ctx* c, int nBCSides, const int* BC, int nBCs, const in
     if (c->S2E[1 + 5 * s] > 0) return 2;  // a boundary side has no neighbour element
     if (BC[s] < 1 || BC[s] > nBCs) return 3;
     const int type = BoundaryType[BC[s] - 1], state = BoundaryType[BC[s] - 1 + nBCs];
-    if (type != 2 && type != 3 && type != 4 && type != 9) return 4;
-    if ((type == 2 || type == 4) && (state < 1 || state > nRefState)) return 5;
+    const bool usesRef = type == 2 || type == 4 || type == 23 || type == 24 || type == 25 || type == 27;
+    if (!usesRef && type != 3 && type != 9 && type != 91 && type != 12 && type != 121) return 4;
+    if (usesRef && (state < 1 || state > nRefState)) return 5;
   }
   c->nBCSides = nBCSides;
   c->nBCs = nBCs;
@@ -833,6 +972,10 @@ int or_set_boundaries(or_ctx* c, int nBCSides, const int* BC, int nBCs, const in
   c->BoundaryType.assign(BoundaryType, BoundaryType + 3 * nBCs);
   c->RefStatePrim.assign(RefStatePrim, RefStatePrim + 6 * nRefState);
   return 0;
+}
+
+void or_set_boundary_data(or_ctx* c, const double* BCDataPrim) {
+  c->BCDataPrim.assign(BCDataPrim, BCDataPrim + 6 * (size_t)c->nf * c->nBCSides);
 }
 
 void or_dg_time_derivative(or_ctx* c, double t, const double* U, double* Ut) {
@@ -913,9 +1056,11 @@ void or_get_density_gradient(or_ctx* c, double* gx, double* gy, double* gz) {
         const int type = bc_type(c, s);
         double Pb[6];
         const double* Pm = &c->UPrim_master[6 * fi];
-        boundary_state(c->e, type, bc_ref(c, s), Pm, &c->NormVec[3 * fi], &c->TangVec1[3 * fi], &c->TangVec2[3 * fi],
+        boundary_state(c->e, type, bc_ref(c, s), bc_data(c, fi), Pm, &c->NormVec[3 * fi], &c->TangVec1[3 * fi],
+                       &c->TangVec2[3 * fi],
                        Pb);
-        const double rb = type == 2 ? 0.5 * (Pm[DENS] + Pb[DENS]) : (type == 9 ? Pm[DENS] : Pb[DENS]);
+        const double rb = riemann_type_bc(type) ? 0.5 * (Pm[DENS] + Pb[DENS])
+                                                : ((type == 9 || type == 91) ? Pm[DENS] : Pb[DENS]);
         f = (rb - Pm[DENS]) * c->SurfElem[fi];
       } else {
         f = -1 * c->UPrim_master[DENS + 6 * fi] + c->UPrim_slave[DENS + 6 * fi];

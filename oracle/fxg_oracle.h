@@ -75,11 +75,22 @@ or_ctx* or_init(const or_params* p, int nElems, int nSides, const int* SideToEle
 void or_free(or_ctx* c);
 /* Boundary sides 1..nBCSides (numbered first, as setLocalSideIDs does, prepare_mesh.f90:196-222):
  * BC(nBCSides) the boundary index of each, BoundaryType(nBCs,3) = (type, state, alpha) per
- * boundary (mesh_readin.f90:167-173), RefStatePrim(6,nRefState) (equation.f90:126-141). Types 2
- * (BCState >= 1), 3, 4 and 9 (getboundaryflux.f90). Returns 0, or a nonzero code for an invalid
- * side, index, type or state. */
+ * boundary (mesh_readin.f90:167-173), RefStatePrim(6,nRefState) (equation.f90:126-141) as InitBC
+ * leaves it. Types 2, 4, 23, 24, 25, 27 (BCState >= 1), 3, 9, 91, 12 and 121
+ * (getboundaryflux.f90). Returns 0, or a nonzero code for an invalid side, index, type or state. */
 int or_set_boundaries(or_ctx* c, int nBCSides, const int* BC, int nBCs, const int* BoundaryType, int nRefState,
                       const double* RefStatePrim);
+/* BCDataPrim(6,0:N,0:N,nBCSides) of the boundaries of types 12 and 121 (equation_vars.f90). */
+void or_set_boundary_data(or_ctx* c, const double* BCDataPrim);
+/* One boundary point (GetBoundaryState, Lifting_GetBoundaryFlux before SurfElem, GetBoundaryFlux
+ * before SurfElem): inner primitive state Pm(6), frame nv/t1/t2, inner gradients gx/gy/gz(4) of
+ * (u,v,w,T); ref = RefStatePrim(:,BCState) or NULL, data = BCDataPrim of the point or NULL. */
+void or_boundary_point(const or_params* p, int type, const double* ref, const double* data, const double* Pm,
+                       const double* nv, const double* t1, const double* t2, const double* gx, const double* gy,
+                       const double* gz, double* Pb, double* Flift, double* Flux);
+/* The gradients a slip wall's (9 or 91) viscous flux sees (getboundaryflux.f90:622-776). */
+void or_slip_gradients(int type, const double* nv, const double* t1, const double* t2, const double* gx,
+                       const double* gy, const double* gz, double* qx, double* qy, double* qz);
 void or_set_threads(int n);
 /* DGTimeDerivative_weakForm (dg.f90:221-678): U(5,0:N,0:N,0:N,nElems) -> Ut. */
 void or_dg_time_derivative(or_ctx* c, double t, const double* U, double* Ut);

@@ -79,6 +79,9 @@ def _setup(L):
     L.or_free.argtypes = [v]
     L.or_set_boundaries.restype = C.c_int
     L.or_set_boundaries.argtypes = [v, C.c_int, _ip, C.c_int, _ip, C.c_int, _dp]
+    L.or_set_boundary_data.argtypes = [v, _dp]
+    L.or_boundary_point.argtypes = [P, C.c_int, v, v, _dp] + [_dp] * 6 + [_dp] * 3
+    L.or_slip_gradients.argtypes = [C.c_int] + [_dp] * 9
     L.or_set_threads.argtypes = [C.c_int]
     L.or_dg_time_derivative.argtypes = [v, C.c_double, _dp, _dp]
     L.or_get_gradients.argtypes = [v, _dp, _dp, _dp]
@@ -167,7 +170,7 @@ def rk_coefs():
 class Solver:
     """The oracle's DG operator on one mesh (FLEXI layout)."""
 
-    def __init__(self, prm: Params, mesh, metrics, ref_state_prim=None, variant="parity"):
+    def __init__(self, prm: Params, mesh, metrics, ref_state_prim=None, variant="parity", bc_data=None):
         self.L = lib(variant)
         self.p = prm
         self.N = prm.N
@@ -187,6 +190,9 @@ class Solver:
                                          bt.shape[0], bt, ref.shape[1], ref)
             if rc != 0:
                 raise ValueError(f"or_set_boundaries: invalid boundary data (code {rc})")
+            if bc_data is not None:
+                self._bc_data = np.asfortranarray(bc_data, dtype=np.float64)
+                self.L.or_set_boundary_data(self.ctx, self._bc_data)
 
     def __del__(self):
         try:
@@ -263,6 +269,28 @@ def surf_int_raw(N, nodeType, S2E, L_HatMinus, L_HatPlus, Fm, Fs, nElems, slave_
                           np.asfortranarray(L_HatMinus), np.asfortranarray(L_HatPlus), nVar,
                           np.asfortranarray(Fm), np.asfortranarray(Fs), slave_sign, Ut)
     return Ut
+
+
+def boundary_point(prm: Params, bc_type, ref, Pm, nv, t1, t2, grads=None, data=None):
+    """One boundary point through GetBoundaryState / Lifting_GetBoundaryFlux / GetBoundaryFlux
+    (before SurfElem): returns (Pb(6), lifting flux(4), flux(5))."""
+    f = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+    g = grads if grads is not None else np.zeros((3, 4))
+    Pb, Fl, Fx = F(6), F(4), F(5)
+    refp = f(ref) if ref is not None else None
+    datp = f(data) if data is not None else None
+    lib().or_boundary_point(C.byref(prm), int(bc_type), refp.ctypes.data if refp is not None else None,
+                            datp.ctypes.data if datp is not None else None, f(Pm), f(nv), f(t1), f(t2),
+                            f(g[0]), f(g[1]), f(g[2]), Pb, Fl, Fx)
+    return Pb, Fl, Fx
+
+
+def slip_gradients(bc_type, nv, t1, t2, grads):
+    """(3,4) gradients of (u,v,w,T) as a slip wall of type 9 or 91 passes them to its viscous flux."""
+    f = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+    q = [F(4) for _ in range(3)]
+    lib().or_slip_gradients(int(bc_type), f(nv), f(t1), f(t2), f(grads[0]), f(grads[1]), f(grads[2]), *q)
+    return np.array(q)
 
 
 def set_threads(n: int, variant: str = "parity") -> None:

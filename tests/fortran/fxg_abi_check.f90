@@ -41,6 +41,7 @@ rk = 0.5_C_DOUBLE
 IF (fxg_destroy(nul).NE.0) STOP 4                      ! destroying NULL is a no-op
 IF (LEN(fxg_string(fxg_pipeline(nul))).NE.0) STOP 5
 IF (fxg_set_boundaries(nul, 1, ibuf, ibuf, 1, dbuf).EQ.0) nbad = nbad + 1
+IF (fxg_set_boundary_data(nul, dbuf).EQ.0) nbad = nbad + 1
 IF (fxg_set_state(nul, dbuf).EQ.0) nbad = nbad + 1
 IF (fxg_get_state(nul, dbuf).EQ.0) nbad = nbad + 1
 IF (fxg_get_ut(nul, dbuf).EQ.0) nbad = nbad + 1

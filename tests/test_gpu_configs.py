@@ -85,14 +85,15 @@ def test_fp32_bulk_fused_split_against_fp64(N):
     assert err_U <= 1e-6
 
 
-@pytest.mark.parametrize("pipeline", ["fused", "classic"])
-def test_curvilinear_freestream_24cubed_n5_gpu_metrics(pipeline, monkeypatch):
+@pytest.mark.parametrize("pipeline,node,split", [("fused", "gauss-lobatto", "pi"), ("classic", "gauss-lobatto", "pi"),
+                                                 ("classic", "gauss-lobatto", None), ("classic", "gauss", None)])
+def test_curvilinear_freestream_24cubed_n5_gpu_metrics(pipeline, node, split, monkeypatch):
     from flexi_amd.solver import DGSolver
     if pipeline == "classic":
         monkeypatch.setenv("FXG_PIPELINE", "classic")
     N = 5
     mesh = periodic_box(24, 24, 24, NGeo=N, deform=sine_deform(0.1))
-    s = DGSolver(mesh, N, node_type="gauss-lobatto", split_dg="pi", riemann="roe")
+    s = DGSolver(mesh, N, node_type=node, split_dg=split, riemann="roe")
     assert s.kernel_pipeline() == pipeline
     n = N + 1
     U0 = testcase.freestream((n, n, n, mesh.nElems), vel=(0.3, 0.2, 0.1), p=1.0 / 1.4)
@@ -100,13 +101,13 @@ def test_curvilinear_freestream_24cubed_n5_gpu_metrics(pipeline, monkeypatch):
     s.run(100)
     U = s.get_state()
     err = np.max(np.abs(U - U0))
-    print(f"freestream 24^3 N=5 curvilinear ({pipeline}), 100 steps to t={s.t:.4f}: max|U-U0| = {err:.3e} "
-          f"(per variable {np.abs(U - U0).reshape(5, -1).max(1)})")
-    # Fused stage with library-computed metrics: each element's face metric is its own boundary
-    # metric, so volume and surface terms cancel to flux round-off. The four-kernel stage follows
-    # FLEXI and gives the slave the master's NormVec*SurfElem, which differs from the slave's own
-    # boundary metric by the round-off of two separate metric evaluations (~2e-12 here).
-    assert err <= (1e-13 if pipeline == "fused" else 1e-11)
+    print(f"freestream 24^3 N=5 curvilinear ({pipeline}, {node}, {split}), 100 steps to t={s.t:.4f}: "
+          f"max|U-U0| = {err:.3e} (per variable {np.abs(U - U0).reshape(5, -1).max(1)})")
+    # With library-computed metrics each element takes its face metric from its own metric terms
+    # (the fused stage from its boundary nodes, the four-kernel stage from its own face
+    # interpolation of the curl-form terms, the slave side included), so an element's volume and
+    # surface terms cancel to flux round-off for a uniform state.
+    assert err <= 1e-13
 
 
 @pytest.mark.parametrize("mesh_source", ["box", "hopr_file"])
