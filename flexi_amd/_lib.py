@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
     L.fxg_timestep_lserkw2.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, dp, dp]
     L.fxg_calc_timestep.argtypes = [vp, C.c_double, C.c_double, dp, ip, vp]
     L.fxg_dg_time_derivative_dev.argtypes = [vp, C.c_double, vp, vp, vp]
+    L.fxg_calc_timestep_dev_noalloc.argtypes = [vp, vp, C.c_double, C.c_double, vp, vp, vp]
     L.fxg_timestep_lserkw2_dev.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, dp, dp, vp, vp, vp]
     L.fxg_rhs_vjp_dev.argtypes = [vp, C.c_double, vp, vp, vp, vp]
     L.fxg_forward_n_steps.argtypes = [vp, C.c_int, dp, C.c_int, dp, dp, dp, vp, vp, C.c_int, vp]
@@ -93,7 +94,7 @@ EXPORTS = ["fxg_last_error", "fxg_version", "fxg_create", "fxg_create_from_nodes
            "fxg_dg_time_derivative",
            "fxg_timestep_lserkw2", "fxg_calc_timestep", "fxg_dg_time_derivative_dev", "fxg_timestep_lserkw2_dev",
            "fxg_rhs_vjp_dev", "fxg_forward_n_steps", "fxg_adjoint_n_steps", "fxg_adjoint_storage", "fxg_pipeline",
-           "fxg_sync", "fxg_sizes",
+           "fxg_sync", "fxg_sizes", "fxg_calc_timestep_dev_noalloc",
            "fxg_profile_enable", "fxg_profile_read"]
 
 KERNEL_NAMES = ["prolong_u", "lift", "flux", "volint", "dt_elem", "dt_reduce", "faces", "visc_faces", "dg_fused"]

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -258,12 +259,48 @@ int upload(fxg_ctx* c, T** dst, const T* src, size_t count) {
 template <class T>
 int alloc_owned(fxg_ctx* c, T** dst, size_t count) { return upload<T>(c, dst, nullptr, count); }
 
+// ---------------------------------------------------------------------------------------------
+// Scratch of fxg_calc_timestep_dev_noalloc: a file-scope device array, so that entry point
+// allocates nothing. Each GPU holds its own copy until the process ends; its address on a device
+// is looked up once, with that device current, when the first context on it is set up.
+// Layout (doubles): a region of 2*kNoAllocMaxElems for dtConv[K] followed by dtVisc[K] (K the
+// context's elements), dtOut[2], then the ints errElem[kNoAllocMaxElems], errOut[1]. Every call
+// writes each entry it reads (k_dt_elem writes all
+// nElems per-element entries, k_dt_reduce the three outputs), so nothing needs clearing; calls that
+// overlap on one GPU share the array and race.
+constexpr int kNoAllocMaxElems = 1 << 19;
+constexpr size_t kNoAllocInts = (size_t)kNoAllocMaxElems + 1;
+constexpr size_t kNoAllocDoubles = 2 * (size_t)kNoAllocMaxElems + 2 + (kNoAllocInts + 1) / 2;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail("device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  double* ptr = nullptr;
+  HIPCHECK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail("hipGetSymbolAddress(scratch) returned NULL");
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
+
+__global__ void k_dt_store(const double* __restrict__ dtOut, const int* __restrict__ errOut, double* __restrict__ dt,
+                           int* __restrict__ errType) {
+  dt[0] = fmin(dtOut[0], dtOut[1]);
+  errType[0] = errOut[0];
+}
+
 // Allocations and tables shared by both creation paths. Geometry arrays are allocated but filled
 // by the caller.
 int ctx_setup(fxg_ctx* c, const fxg_params* p, int K, int S, const int* S2E, const int* E2S, int nBC) {
   c->p = *p;
   c->nBC = nBC;
   HIPCHECK(hipGetDevice(&c->device));
+  if (scratch_init(c->device)) return 1;
   c->N = p->N; c->n = p->N + 1; c->nf = c->n * c->n; c->np = c->nf * c->n; c->K = K; c->S = S;
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const double kappaM1 = p->kappa - 1.0;
@@ -994,6 +1031,27 @@ int fxg_calc_timestep(fxg_ctx* c, double CFL, double DFL, double* dt, int* errTy
   *errType = c->errHost[0];
   if (*errType != 0) return fail("CALCTIMESTEP: NaN in density (1), convective (2) or viscous (3) time step: " +
                                  std::to_string(*errType));
+  return 0;
+}
+
+int fxg_calc_timestep_dev_noalloc(fxg_ctx* c, const double* U, double CFL, double DFL, double* dt, int* errType,
+                                  void* stream) {
+  if (!c || !U || !dt || !errType) return fail("NULL argument");
+  if (c->K > kNoAllocMaxElems)
+    return fail("fxg_calc_timestep_dev_noalloc: " + std::to_string(c->K) + " elements exceed its scratch maximum of " +
+                std::to_string(kNoAllocMaxElems));
+  double* p = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if (!p) return fail("fxg_calc_timestep_dev_noalloc: no scratch address for the context's device");
+  // the per-element and reduced values in the scratch array, in the layout described at `scratch`
+  fxg::DevWork w{};
+  w.dtElem = p;
+  w.dtOut = p + 2 * (size_t)kNoAllocMaxElems;
+  w.errElem = reinterpret_cast<int*>(w.dtOut + 2);
+  w.errOut = w.errElem + kNoAllocMaxElems;
+  const hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  HIPCHECK(fxg::launch_calc_timestep(c->m, w, c->gas, U, CFL, DFL, s, nullptr));
+  hipLaunchKernelGGL(k_dt_store, dim3(1), dim3(1), 0, s, w.dtOut, w.errOut, dt, errType);
+  HIPCHECK(hipGetLastError());
   return 0;
 }
 

@@ -38,6 +38,7 @@ PUBLIC :: fxg_set_state, fxg_get_state, fxg_get_ut, fxg_get_gradients, fxg_get_m
 PUBLIC :: fxg_tgv_analyze
 PUBLIC :: fxg_dg_time_derivative, fxg_timestep_lserkw2, fxg_calc_timestep
 PUBLIC :: fxg_dg_time_derivative_dev, fxg_timestep_lserkw2_dev, fxg_pipeline, fxg_sync, fxg_sizes
+PUBLIC :: fxg_calc_timestep_dev_noalloc
 PUBLIC :: fxg_rhs_vjp_dev, fxg_forward_n_steps, fxg_adjoint_n_steps, fxg_adjoint_storage
 PUBLIC :: fxg_profile_enable, fxg_profile_read, fxg_string
 
@@ -141,6 +142,12 @@ INTERFACE
     IMPORT :: C_INT, C_DOUBLE, C_PTR
     TYPE(C_PTR),VALUE    :: ctx, U_dev, Ut_dev, stream
     REAL(C_DOUBLE),VALUE :: t
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION fxg_calc_timestep_dev_noalloc(ctx,U_dev,CFLScale,DFLScale,dt_dev,errType_dev,stream) &
+      BIND(C,NAME='fxg_calc_timestep_dev_noalloc')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR),VALUE    :: ctx, U_dev, dt_dev, errType_dev, stream
+    REAL(C_DOUBLE),VALUE :: CFLScale, DFLScale
   END FUNCTION
   INTEGER(C_INT) FUNCTION fxg_timestep_lserkw2_dev(ctx,t,dt,nStages,RKA,RKb,RKc,U_dev,R_dev,stream) &
       BIND(C,NAME='fxg_timestep_lserkw2_dev')

@@ -206,6 +206,18 @@ int fxg_adjoint_n_steps(fxg_ctx* ctx, int nSteps, const double* dt, int nStages,
                         const double* RKb, const double* RKc, const double* U0_dev, const double* ckpt_dev,
                         double* ubar_dev, int checkpointEvery, void* stream);
 int fxg_adjoint_storage(fxg_ctx* ctx, int nSteps, int nStages, int checkpointEvery, size_t* doubles);
+/* CALCTIMESTEP of a device state without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same arithmetic as fxg_calc_timestep on U_dev (FLEXI
+ * layout, the context's device, which must be current): *dt_dev = MIN(convective, viscous) time
+ * step and *errType_dev = errType (0 ok, 1 NaN density, 2 NaN convective, 3 NaN viscous dt; the
+ * call itself does not fail on a NaN), both in device memory the caller owns. Asynchronous on
+ * `stream` (NULL = the context's). Its per-element scratch is a fixed device array of the library
+ * (about 10 MB per GPU, held until the process ends) for meshes of at most 524,288 elements;
+ * larger meshes are rejected. All calls on one GPU share that array: calls that overlap on one
+ * GPU (other streams, other threads, other contexts) race and must be serialised by the caller. */
+int fxg_calc_timestep_dev_noalloc(fxg_ctx* ctx, const double* U_dev, double CFLScale, double DFLScale,
+                                  double* dt_dev, int* errType_dev, void* stream);
 /* Which stage kernels the context runs: "fused" (Gauss-Lobatto split form, N 2..7: two
  * kernels per stage, face states carried between LSERK stages) or "classic" (prolongation,
  * lifting, side fluxes, integrals). No counterpart in the reference; FXG_PIPELINE=classic in

@@ -52,6 +52,7 @@ IF (fxg_dg_time_derivative(nul, 0._C_DOUBLE).EQ.0) nbad = nbad + 1
 IF (fxg_timestep_lserkw2(nul, 0._C_DOUBLE, 1._C_DOUBLE, 5, rk, rk, rk).EQ.0) nbad = nbad + 1
 IF (fxg_calc_timestep(nul, 0.9_C_DOUBLE, 0.9_C_DOUBLE, dval, rc, nul).EQ.0) nbad = nbad + 1
 IF (fxg_dg_time_derivative_dev(nul, 0._C_DOUBLE, nul, nul, nul).EQ.0) nbad = nbad + 1
+IF (fxg_calc_timestep_dev_noalloc(nul, nul, 1._C_DOUBLE, 1._C_DOUBLE, nul, nul, nul).EQ.0) nbad = nbad + 1
 IF (fxg_timestep_lserkw2_dev(nul, 0._C_DOUBLE, 1._C_DOUBLE, 5, rk, rk, rk, nul, nul, nul).EQ.0) nbad = nbad + 1
 IF (fxg_rhs_vjp_dev(nul, 0._C_DOUBLE, nul, nul, nul, nul).EQ.0) nbad = nbad + 1
 IF (fxg_forward_n_steps(nul, 1, rk, 5, rk, rk, rk, nul, nul, 1, nul).EQ.0) nbad = nbad + 1
