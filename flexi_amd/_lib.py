@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
 class FxgParams(C.Structure):
     _fields_ = [("N", C.c_int), ("nodeType", C.c_int), ("splitDG", C.c_int), ("riemann", C.c_int),
                 ("kappa", C.c_double), ("R", C.c_double), ("Pr", C.c_double), ("mu0", C.c_double),
-                ("doWeakLifting", C.c_int), ("doConservativeLifting", C.c_int), ("precision", C.c_int)]
+                ("doWeakLifting", C.c_int), ("doConservativeLifting", C.c_int), ("precision", C.c_int),
+                ("metrics", C.c_int)]
 
 
 class FxgMesh(C.Structure):

@@ -209,6 +209,8 @@ int check_params(const fxg_params* p) {
   if (p->doWeakLifting || p->doConservativeLifting)
     return fail("only the strong, non-conservative BR1 lifting is built (doWeakLifting=F, doConservativeLifting=F)");
   if (p->precision != FXG_PREC_FP64 && p->precision != FXG_PREC_FP32_BULK) return fail("unknown precision");
+  if (p->metrics != FXG_METRICS_OWN && p->metrics != FXG_METRICS_REFERENCE)
+    return fail("metrics must be FXG_METRICS_OWN (0) or FXG_METRICS_REFERENCE (1)");
   if (!(p->kappa > 1.0) || !(p->R > 0.0) || !(p->Pr > 0.0) || !(p->mu0 >= 0.0)) return fail("invalid gas constants");
   return 0;
 }
@@ -528,21 +530,23 @@ int fxg_create_from_nodes(const fxg_params* p, int nElems, int nSides, const int
   double* dNC = nullptr;
   if (upload(c, &dNC, NodeCoords, 3 * ng * nElems)) { ctx_free(c); return 1; }
   std::string err;
+  const bool own = p->metrics == FXG_METRICS_OWN;
   // four-kernel path: each side of a face takes its own face metric too (the slave's in slaveNS,
   // its side fluxes in FluxS), as the fused path does with its boundary nodes
   double* slaveNS = nullptr;
-  if (!c->fused) {
+  if (own && !c->fused) {
     const size_t face = (size_t)c->nf * nSides;
     if (alloc_owned(c, &slaveNS, 3 * face) || alloc_owned(c, &c->w.FluxS, 5 * face)) { ctx_free(c); return 1; }
   }
   if (fxg::compute_metrics_gpu(c->N, p->nodeType, NGeo, nElems, dNC, c->m, c->Mf, c->Mg, c->Mh, c->sJ, c->xGP, c->NV,
-                               c->T1, c->T2, c->SE, slaveNS, c->stream, err)) {
+                               c->T1, c->T2, c->SE, slaveNS, !own, c->stream, err)) {
     fail(err); ctx_free(c); return 1;
   }
   c->m.slaveNS = slaveNS;
-  // the curl-form face metric depends only on the face, so each element's own boundary metric
-  // is the face metric; using it on both sides keeps the freestream exact to flux round-off
-  c->m.ownFaceMetric = 1;
+  // FXG_METRICS_OWN: the curl-form face metric depends only on the face, so each element's own
+  // boundary metric is the face metric; using it on both sides keeps the freestream exact to
+  // flux round-off. FXG_METRICS_REFERENCE: the master's NormVec/SurfElem on both sides, as FLEXI.
+  c->m.ownFaceMetric = own ? 1 : 0;
   if (finish_setup(c)) { ctx_free(c); return 1; }
   *out = c;
   return 0;

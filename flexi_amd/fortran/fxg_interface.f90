@@ -18,11 +18,14 @@ INTEGER(C_INT),PARAMETER,PUBLIC :: FXG_SPLITDG_NONE          = 0   ! splitflux.f
 INTEGER(C_INT),PARAMETER,PUBLIC :: FXG_SPLITDG_PI            = 4
 INTEGER(C_INT),PARAMETER,PUBLIC :: FXG_PREC_FP64             = 0
 INTEGER(C_INT),PARAMETER,PUBLIC :: FXG_PREC_FP32_BULK        = 1
+INTEGER(C_INT),PARAMETER,PUBLIC :: FXG_METRICS_OWN           = 0
+INTEGER(C_INT),PARAMETER,PUBLIC :: FXG_METRICS_REFERENCE     = 1
 
 TYPE,BIND(C),PUBLIC :: fxg_params
   INTEGER(C_INT) :: N, nodeType, splitDG, riemann
   REAL(C_DOUBLE) :: kappa, R, Pr, mu0
   INTEGER(C_INT) :: doWeakLifting, doConservativeLifting, precision
+  INTEGER(C_INT) :: metrics
 END TYPE fxg_params
 
 TYPE,BIND(C),PUBLIC :: fxg_mesh

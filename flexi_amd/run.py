@@ -16,9 +16,14 @@ uniform RefState for IniExactFunc = 1), kappa, R, Pr, mu0, TStart, TEnd, CFLScal
 Analyze_dt, nWriteData, maxIter, TimeDiscMethod, nCalcTimeStepMax (only 1), MachNumber,
 IniConstDens (T only), NAnalyze and nAnalyzeTestCase of the TGV testcase (whose analysis goes to
 ``<ProjectName>_TGVAnalysis.csv`` as FLEXI writes it), BoundaryName/BoundaryType (boundary types
-2 with a RefState, or with BCState 0 and IniExactFunc = 1; 3, 4, 9). The node type is a
-compile-time choice in FLEXI (FLEXI_NODETYPE); here it is ``--node-type``. Other keys are
-reported and ignored.
+2, 23, 24, 25, 27 with a RefState, or 2 with BCState 0 and IniExactFunc = 1; 3, 4, 9, 91). The
+node type is a compile-time choice in FLEXI (FLEXI_NODETYPE); here it is ``--node-type``. Other
+keys are reported and ignored.
+
+The metric terms are computed on the GPU in metrics.f90's operation order with FLEXI's side
+normals shared by both sides (``--metric-order reference``, the default), so that a run follows
+FLEXI's own to round-off; ``--metric-order own`` takes the library's metrics, which keep the
+freestream exact on curved meshes.
 """
 from __future__ import annotations
 
@@ -149,7 +154,8 @@ class Case:
         return testcase.tgv(xgp, kappa=self.kappa, R=self.R, Ma=self.mach)
 
 
-def run(case: Case, out_dir: str, precision: str = "fp64", restart: str | None = None, log=print) -> dict:
+def run(case: Case, out_dir: str, precision: str = "fp64", restart: str | None = None, log=print,
+        metric_order: str = "reference") -> dict:
     mesh = io.read_mesh_bc(case.mesh_path, case.boundaries)
     ref_prim = equation.ref_state_prim(case.ref_states, case.R) if case.ref_states else None
     if mesh.nBCSides and np.any((mesh.BoundaryType[:, 0] == 2) & (mesh.BoundaryType[:, 1] == 0)):
@@ -173,7 +179,7 @@ def run(case: Case, out_dir: str, precision: str = "fp64", restart: str | None =
             ref_prim[1:4, st - 1] = (1.0 / nrm, ta / nrm, tb / nrm)
     s = DGSolver(mesh, case.N, node_type=case.node_type, split_dg=case.split, riemann=case.riemann,
                  kappa=case.kappa, R=case.R, Pr=case.Pr, mu0=case.mu0, precision=precision, scheme=case.scheme,
-                 CFLScale=case.CFL, DFLScale=case.DFL, ref_state_prim=ref_prim)
+                 CFLScale=case.CFL, DFLScale=case.DFL, ref_state_prim=ref_prim, metric_order=metric_order)
     try:
         return _time_loop(case, s, mesh.nElems, out_dir, restart, log)
     finally:
@@ -262,12 +268,13 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32-bulk"])
     ap.add_argument("--restart", default=None, help="State file to start from")
     ap.add_argument("--out-dir", default=".")
+    ap.add_argument("--metric-order", default="reference", choices=["reference", "own"])
     args = ap.parse_args(argv)
     prm = parse_parameter_file(args.parameter_file)
     case = Case(prm, os.path.dirname(os.path.abspath(args.parameter_file)), args.node_type)
     if case.ignored:
         print("ignored parameters: " + ", ".join(case.ignored), file=sys.stderr)
-    res = run(case, args.out_dir, args.precision, args.restart)
+    res = run(case, args.out_dir, args.precision, args.restart, metric_order=args.metric_order)
     print(f"t = {res['t']:.10g} after {res['iterations']} iterations; states: {', '.join(res['states'])}")
     return 0
 

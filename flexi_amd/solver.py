@@ -25,6 +25,7 @@ NODE_TYPES = {"gauss": 1, "gauss-lobatto": 2, 1: 1, 2: 2}
 RIEMANN = {"lf": 1, "roe": 3, "roeentropyfix": 33, 1: 1, 3: 3, 33: 33}
 SPLITDG = {None: 0, "none": 0, "pi": 4, 0: 0, 4: 4}
 PRECISION = {"fp64": 0, "fp32-bulk": 1, "fp32": 1, 0: 0, 1: 1}
+METRIC_ORDER = {"own": 0, "reference": 1, 0: 0, 1: 1}
 
 
 def _f(a, dtype=np.float64):
@@ -41,7 +42,8 @@ class DGSolver:
     def __init__(self, mesh: Mesh, N: int, node_type="gauss-lobatto", split_dg="pi", riemann="roe",
                  kappa=1.4, R=71.42857, Pr=0.72, mu0=6.25e-4, precision="fp64", metrics: Optional[dict] = None,
                  scheme: LSERK = CARPENTER_RK4_5, CFLScale=0.9, DFLScale=0.9,
-                 ref_state_prim: Optional[np.ndarray] = None, bc_data: Optional[np.ndarray] = None):
+                 ref_state_prim: Optional[np.ndarray] = None, bc_data: Optional[np.ndarray] = None,
+                 metric_order: str = "own"):
         L = _lib.lib()
         self.mesh = mesh
         self.N = N
@@ -52,8 +54,10 @@ class DGSolver:
         self.kappa, self.R, self.Pr, self.mu0 = kappa, R, Pr, mu0
         self.scheme = scheme
         self.CFL, self.DFL = fill_cfl_dfl(N, self.node_type, CFLScale, DFLScale, scheme)
+        # metric terms computed by the library (no `metrics`): "own" (FXG_METRICS_OWN) or "reference"
+        # (FXG_METRICS_REFERENCE, metrics.f90's operation order and FLEXI's shared side normals)
         self.params = _lib.FxgParams(N, self.node_type, self.split, self.riemann, kappa, R, Pr, mu0, 0, 0,
-                                     self.precision)
+                                     self.precision, METRIC_ORDER[metric_order])
         self.ctx = C.c_void_p()
         s2e, e2s = _f(mesh.SideToElem, np.int32), _f(mesh.ElemToSide, np.int32)
         if metrics is None:

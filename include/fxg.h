@@ -36,6 +36,16 @@ extern "C" {
  * and the Jacobian stay FP64 in both modes. */
 #define FXG_PREC_FP64 0
 #define FXG_PREC_FP32_BULK 1
+/* Metric terms of fxg_create_from_nodes. FXG_METRICS_OWN: the curl form with coordinates relative
+ * to each element's first node, and every element takes its own boundary metric as the face
+ * metric of its sides: the freestream is kept to flux round-off on curved meshes. Use it unless a
+ * run must follow FLEXI's. FXG_METRICS_REFERENCE: CalcMetrics/CalcSurfMetrics in
+ * metrics.f90's operation order (absolute coordinates, one-dimensional basis changes in
+ * sequence, no FMA contraction), with the master's NormVec/SurfElem shared by both sides as
+ * in FLEXI: the metric terms FLEXI computes, to the rounding of the basis matrices. fxg_create
+ * (metrics from the host) always shares the side's NormVec/SurfElem. */
+#define FXG_METRICS_OWN 0
+#define FXG_METRICS_REFERENCE 1
 
 /* Run-time parameters. Names and meanings follow FLEXI's parameter file keys:
  * N, NodeType (compile-time PP_NodeType in FLEXI), SplitDG (src/equations/navierstokes/
@@ -50,6 +60,7 @@ typedef struct {
   int doWeakLifting;         /* only 0 (strong BR1) is supported */
   int doConservativeLifting; /* only 0 (non-conservative volume integral) is supported */
   int precision;             /* FXG_PREC_* */
+  int metrics;               /* FXG_METRICS_*: how fxg_create_from_nodes forms the metric terms */
 } fxg_params;
 
 /* Mesh and metric arrays as FLEXI holds them after InitMesh (src/mesh/mesh_vars.f90:26-209):
@@ -92,7 +103,8 @@ const char* fxg_version(void);
 int fxg_create(const fxg_params* params, const fxg_mesh* mesh, fxg_ctx** ctx);
 /* Set-up from geometry: NodeCoords(3,0:NGeo,0:NGeo,0:NGeo,nElems) on equidistant nodes; the
  * metrics of CalcMetrics/CalcSurfMetrics (src/mesh/metrics.f90:161-787, curl form) are computed
- * on the GPU. SideToElem/ElemToSide as in fxg_mesh, boundary sides first. */
+ * on the GPU as params->metrics selects. SideToElem/ElemToSide as in fxg_mesh, boundary sides
+ * first. */
 int fxg_create_from_nodes(const fxg_params* params, int nElems, int nSides,
                           const int* SideToElem, const int* ElemToSide, int NGeo,
                           const double* NodeCoords, fxg_ctx** ctx);

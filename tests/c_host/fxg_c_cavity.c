@@ -73,7 +73,7 @@ int main(int argc, char** argv) {
     return fail_io("read mesh");
 
   fxg_params p = {N, FXG_NODE_GAUSS, FXG_SPLITDG_NONE, FXG_RIEMANN_ROEENTROPYFIX, kappa, R, 0.72, 0.01, 0, 0,
-                  FXG_PREC_FP64};
+                  FXG_PREC_FP64, FXG_METRICS_REFERENCE};
   fxg_ctx* ctx = NULL;
   if (fxg_create_from_nodes(&p, K, S, s2e, e2s, NGeo, nodes, &ctx)) return fail_fxg("create");
 

@@ -46,7 +46,7 @@ int main(int argc, char** argv) {
   if (fxg_io_read_mesh(meshFile, K, S, NGeo, s2e, e2s, nodes)) return fail_io("read mesh");
 
   fxg_params p = {N, FXG_NODE_GAUSS_LOBATTO, FXG_SPLITDG_PI, FXG_RIEMANN_ROEENTROPYFIX,
-                  1.4, 71.42857, 0.72, 6.25e-4, 0, 0, FXG_PREC_FP64};
+                  1.4, 71.42857, 0.72, 6.25e-4, 0, 0, FXG_PREC_FP64, FXG_METRICS_OWN};
   fxg_ctx* ctx = NULL;
   if (fxg_create_from_nodes(&p, K, S, s2e, e2s, NGeo, nodes, &ctx)) return fail_fxg("create");
 

@@ -20,7 +20,7 @@ WRITE(*,'(A,I0)') 'sizeof(fxg_params)=', C_SIZEOF(prm)
 WRITE(*,'(A,I0)') 'sizeof(fxg_mesh)=', C_SIZEOF(mesh)
 WRITE(*,'(2A)') 'version=', fxg_string(fxg_version())
 prm = fxg_params(9, FXG_NODE_GAUSS_LOBATTO, FXG_SPLITDG_PI, FXG_RIEMANN_ROE, 1.4_C_DOUBLE, 287._C_DOUBLE, &
-                 0.72_C_DOUBLE, 1.E-3_C_DOUBLE, 0, 0, FXG_PREC_FP64)
+                 0.72_C_DOUBLE, 1.E-3_C_DOUBLE, 0, 0, FXG_PREC_FP64, FXG_METRICS_OWN)
 mesh%nElems = 1; mesh%nSides = 3; mesh%firstInnerSide = 1; mesh%lastInnerSide = 3
 rc = fxg_create(prm, mesh, ctx)
 msg = fxg_string(fxg_last_error())

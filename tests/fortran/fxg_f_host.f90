@@ -77,7 +77,7 @@ IF (fxg_io_read_mesh(TRIM(meshFile)//C_NULL_CHAR, nElems, nSides, NGeo, SideToEl
   CALL AbortIO('read mesh')
 
 prm = fxg_params(N, FXG_NODE_GAUSS_LOBATTO, FXG_SPLITDG_PI, FXG_RIEMANN_ROEENTROPYFIX, 1.4_C_DOUBLE, &
-                 71.42857_C_DOUBLE, 0.72_C_DOUBLE, 6.25E-4_C_DOUBLE, 0, 0, FXG_PREC_FP64)
+                 71.42857_C_DOUBLE, 0.72_C_DOUBLE, 6.25E-4_C_DOUBLE, 0, 0, FXG_PREC_FP64, FXG_METRICS_OWN)
 IF (fxg_create_from_nodes(prm, nElems, nSides, SideToElem, ElemToSide, NGeo, NodeCoords, ctx).NE.0) &
   CALL AbortFXG('create')
 IF (fxg_io_read_state(TRIM(stateIn)//C_NULL_CHAR, 5, N, nElems, U, t, nodeType, LEN(nodeType), &

@@ -90,6 +90,36 @@ def test_gpu_metrics_match_oracle(N, NGeo, node):
         assert rel(g[k], m[k]) <= 1e-12, (k, rel(g[k], m[k]))
 
 
+@pytest.mark.parametrize("N,NGeo,node", [(5, 5, orc.NODE_GL), (3, 2, orc.NODE_G), (7, 1, orc.NODE_GL), (4, 6, orc.NODE_GL),
+                                         (7, 0, orc.NODE_GL)])
+def test_gpu_reference_order_metrics_match_oracle(N, NGeo, node):
+    """FXG_METRICS_REFERENCE: the metric terms in metrics.f90's operation order on the GPU, against
+    the oracle, which follows the same order. What remains is the last bit of a few entries of the
+    basis matrices (the library builds its Gauss and Gauss-Lobatto nodes by its own Newton
+    iteration): measured at most 4.6e-15 of an array's largest magnitude (sJ of the NGeo = 5
+    mesh), 0 on most arrays. The library's own order (FXG_METRICS_OWN) differs from the oracle by
+    up to 1.5e-13 on the same meshes. Bar: 1e-14 of each array's largest magnitude. NGeo 0 stands
+    for the tgv/split case's own HOPR mesh (8^3, NGeo 1)."""
+    from flexi_amd.solver import DGSolver
+    if NGeo == 0:
+        import os
+        from flexi_amd import io
+        mesh = io.read_mesh(os.path.join(os.path.dirname(__file__), "golden", "hdf5", "CART_HEX_PERIODIC_008_mesh.h5"))
+    else:
+        mesh = periodic_box(3, 2, 2, NGeo=NGeo, deform=sine_deform(0.1))
+    m = orc.calc_metrics(N, node, mesh)
+    gpu = DGSolver(mesh, N, node_type=node, split_dg="pi" if node == orc.NODE_GL else None, riemann="roe",
+                   metric_order="reference")
+    g = gpu.metrics()
+    gpu.close()
+    worst = {}
+    for k in ("Metrics_fTilde", "Metrics_gTilde", "Metrics_hTilde", "sJ", "NormVec", "TangVec1", "TangVec2",
+              "SurfElem", "Elem_xGP"):
+        worst[k] = float(np.max(np.abs(g[k] - m[k])) / np.max(np.abs(m[k])))
+    print(f"reference-order metrics N={N} NGeo={NGeo}: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+    assert max(worst.values()) <= 1e-14, worst
+
+
 def test_calc_timestep_matches_oracle():
     mesh, m, oracle, gpu, U = setup(7, orc.NODE_GL, True, orc.RIEMANN_ROE)
     gpu.set_state(U)
