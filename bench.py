@@ -48,7 +48,12 @@ B_STAGE = 432.0
 # HBM bytes per DOF of one launch of the fused kernels at this workload (FP64, mean over the 5
 # stages of a step), from rocprofv3 FETCH_SIZE (x2.0, plain-copy calibration) + WRITE_SIZE runs of
 # this benchmark; the summary and the commands are in profiles/fused_stage_tgv32_n7_fp64.md
-MEASURED_BYTES_PER_DOF = {"dg_fused": 420.0, "visc_faces": 140.6, "faces": 61.3}
+MEASURED_BYTES_PER_DOF = {"dg_fused": 364.1, "visc_faces": 143.3, "faces": 64.0}
+# FP64 floating-point operations per DOF of one launch (mean over the 5 stages of a step): rocprofv3
+# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 per dispatch, wave instructions x 64 lanes (every wave of
+# these kernels runs with all 64 lanes or skips a phase as a whole), FMA counted as 2; same summary
+MEASURED_FLOP_PER_DOF = {"dg_fused": 2219.6, "visc_faces": 413.7}
+FP64_PEAK_TFLOPS = 78.6
 RHS_EVALS = 1000
 
 
@@ -450,7 +455,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (MEASURED_BYTES_PER_DOF[dom] * r64["dof"]
                                  if dom in MEASURED_BYTES_PER_DOF and args.elems == 32 else None),
-                     "algorithmic_bytes_per_dof": ALGO_BYTES_PER_DOF[dom], "avg_launch_ms": avg_s * 1e3},
+                     "algorithmic_bytes_per_dof": ALGO_BYTES_PER_DOF[dom], "avg_launch_ms": avg_s * 1e3,
+                     "flops": ({"achieved": MEASURED_FLOP_PER_DOF[dom] * r64["dof"] / avg_s / 1e12,
+                                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": MEASURED_FLOP_PER_DOF[dom] * r64["dof"] / avg_s / 1e12 / FP64_PEAK_TFLOPS,
+                                "flop_per_dof": MEASURED_FLOP_PER_DOF[dom],
+                                "source": "rocprofv3 SQ_INSTS_VALU_*_F64 x 64 lanes, FMA = 2"}
+                               if dom in MEASURED_FLOP_PER_DOF else None)},
         "roofline_stage": {"bytes_per_dof_stage": B_STAGE, "stage_ms": stage_s * 1e3, "achieved": stage_achieved,
                            "unit": "GB/s", "frac": stage_achieved / HBM_PEAK_GBS,
                            "dof_stage_per_s_at_peak": HBM_PEAK_GBS * 1e9 / B_STAGE},
