@@ -171,7 +171,8 @@ __device__ __forceinline__ double face_frame(const DevMesh& m, int e, int ls, in
     const double* Jd = (dir == 0 ? m.Mf : (dir == 1 ? m.Mg : m.Mh)) + 3 * ((size_t)np * e + id);
     const double ns0 = sgn * Jd[0], ns1 = sgn * Jd[1], ns2 = sgn * Jd[2];
     se = sqrt(ns0 * ns0 + ns1 * ns1 + ns2 * ns2);
-    nv[0] = (RF)(ms * ns0 / se); nv[1] = (RF)(ms * ns1 / se); nv[2] = (RF)(ms * ns2 / se);
+    const double mse = ms / se;
+    nv[0] = (RF)(ns0 * mse); nv[1] = (RF)(ns1 * mse); nv[2] = (RF)(ns2 * mse);
   } else {
     const size_t sp = slot_side_point<N>(own);
     se = m.SurfElem[sp];
@@ -182,8 +183,8 @@ __device__ __forceinline__ double face_frame(const DevMesh& m, int e, int ls, in
   if (ax <= ay && ax <= az) h[0] = RF(1); else if (ay <= az) h[1] = RF(1); else h[2] = RF(1);
   const RF dt_ = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
   RF tt[3] = {h[0] - dt_ * nv[0], h[1] - dt_ * nv[1], h[2] - dt_ * nv[2]};
-  const RF tn = sqrt(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
-  t1[0] = tt[0] / tn; t1[1] = tt[1] / tn; t1[2] = tt[2] / tn;
+  const RF stn = RF(1) / sqrt(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
+  t1[0] = tt[0] * stn; t1[1] = tt[1] * stn; t1[2] = tt[2] * stn;
   t2[0] = nv[1] * t1[2] - nv[2] * t1[1];
   t2[1] = nv[2] * t1[0] - nv[0] * t1[2];
   t2[2] = nv[0] * t1[1] - nv[1] * t1[0];

@@ -84,12 +84,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
   const int e = blockIdx.x, t = threadIdx.x;
   const bool act = t < np;
   const size_t gid = (size_t)np * e + (act ? t : 0);
-  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
+  // the basis, with the DVolSurf block halved (exact) for the split-flux loop
+  for (int x = t; x < 3 * n * n + 4 * n; x += NT) {
+    const double b = m.basis[x];
+    sB[x] = (x >= B.DVolSurf() && x < B.DVolSurf() + n * n) ? 0.5 * b : b;
+  }
   if (act) {
     const double* c = U + 5 * gid;
     const Prim q = cons_to_prim(c, gas);
     const double E = c[4];
-    sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (E + q.p) / q.rho;
+    sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (E + q.p) * q.srho;
     sA[3 * np + t] = q.u; sA[4 * np + t] = q.v; sA[5 * np + t] = q.w; sA[6 * np + t] = q.T; sA[7 * np + t] = E;
   }
   for (int x = t; x < NF; x += NT) {
@@ -125,7 +129,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
       const double* Jd = (dir == 0 ? m.Mf : (dir == 1 ? m.Mg : m.Mh)) + 3 * ((size_t)np * e + id);
       const double ns0 = sgn * Jd[0], ns1 = sgn * Jd[1], ns2 = sgn * Jd[2];
       se = sqrt(ns0 * ns0 + ns1 * ns1 + ns2 * ns2);
-      nv[0] = (RF)(ms * ns0 / se); nv[1] = (RF)(ms * ns1 / se); nv[2] = (RF)(ms * ns2 / se);
+      const double mse = ms / se;
+      nv[0] = (RF)(ns0 * mse); nv[1] = (RF)(ns1 * mse); nv[2] = (RF)(ns2 * mse);
     } else {
       // the side's normal and surface element, shared by master and slave as in FLEXI
       const size_t sp = slot_side_point<N>(own);
@@ -140,8 +145,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
       if (ax <= ay && ax <= az) h[0] = RF(1); else if (ay <= az) h[1] = RF(1); else h[2] = RF(1);
       const RF dt_ = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
       RF tt[3] = {h[0] - dt_ * nv[0], h[1] - dt_ * nv[1], h[2] - dt_ * nv[2]};
-      const RF tn = sqrt(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
-      t1[0] = tt[0] / tn; t1[1] = tt[1] / tn; t1[2] = tt[2] / tn;
+      const RF stn = RF(1) / sqrt(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
+      t1[0] = tt[0] * stn; t1[1] = tt[1] * stn; t1[2] = tt[2] * stn;
       t2[0] = nv[1] * t1[2] - nv[2] * t1[1];
       t2[1] = nv[2] * t1[0] - nv[0] * t1[2];
       t2[2] = nv[0] * t1[1] - nv[1] * t1[0];
@@ -226,7 +231,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
   __syncthreads();  // the face rows are free: the metric terms take their place
 
   // split-form volume integral with the viscous weak-form part (volint.f90:211-371)
-  const double* DV = sB + B.DVolSurf();
+  const double* DVh = sB + B.DVolSurf();  // DVolSurf/2
   const double* DH = sB + B.D_Hat_T();
   RF rho = 0, uu = 0, vv = 0, ww = 0, pp = 0, HH = 0;
   RF* sJa = sL + L::JA;
@@ -277,17 +282,21 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
         const RF us = uu + (RF)sA[3 * np + id], vs = vv + (RF)sA[4 * np + id], ws = ww + (RF)sA[5 * np + id];
         const RF ps = pp + (RF)sA[np + id], Hs = HH + (RF)sA[2 * np + id];
         // Pirozzoli two-point flux with the metric average a = b/2 (SplitVolumeFluxPI,
-        // splitflux.f90:603-666), evaluated as 2F with b = Ja_i + Ja_l and weighted by DV/2: the
-        // powers of two make every product and sum round exactly as in F and DV*F
+        // splitflux.f90:603-666), weighted by DV: with b = Ja_i + Ja_l, h = DV/2, Qh = h rs (b.u_sum)/4
+        // and P = h p_sum, DV*F = (2 Qh, Qh u_sum + b P, Qh H_sum)
         const RF b0 = m0 + (RF)sM[id], b1 = m1 + (RF)sM[np + id], b2 = m2 + (RF)sM[2 * np + id];
         const RF mb = b0 * us + b1 * vs + b2 * ws;  // 2 (a . u_sum)
-        const RF Q = RF(0.25) * rs * mb;           // 2 qq = rs (a . u_sum) / 2
-        const RF dv = (RF)DV[l + n * me], dvh = RF(0.5) * dv;
-        acc[0] += dv * Q;
-        acc[1] += dvh * (Q * us + b0 * ps);
-        acc[2] += dvh * (Q * vs + b1 * ps);
-        acc[3] += dvh * (Q * ws + b2 * ps);
-        acc[4] += dvh * (Q * Hs);
+        const RF h = (RF)DVh[l + n * me];
+        const RF Qh = h * (RF(0.25) * rs * mb);
+        const RF P = h * ps;
+        acc[0] += RF(2) * Qh;
+        acc[1] += Qh * us;
+        acc[1] += b0 * P;
+        acc[2] += Qh * vs;
+        acc[2] += b1 * P;
+        acc[3] += Qh * ws;
+        acc[3] += b2 * P;
+        acc[4] += Qh * Hs;
       }
     }
     __syncthreads();

@@ -25,7 +25,7 @@ struct GasT {
 // (idealgas/eos.f90:203-230). Always evaluated in FP64: the pressure is a difference of two
 // large numbers at low Mach number.
 struct Prim {
-  double rho, u, v, w, p, T;
+  double rho, u, v, w, p, T, srho;  // srho = 1/rho
 };
 
 __device__ __forceinline__ Prim cons_to_prim(const double* c, const Gas& g) {
@@ -36,7 +36,8 @@ __device__ __forceinline__ Prim cons_to_prim(const double* c, const Gas& g) {
   q.v = c[2] * sRho;
   q.w = c[3] * sRho;
   q.p = g.kappaM1 * (c[4] - 0.5 * (c[1] * q.u + c[2] * q.v + c[3] * q.w));
-  q.T = q.p * sRho / g.R;
+  q.T = q.p * sRho * g.sR;
+  q.srho = sRho;
   return q;
 }
 
@@ -87,8 +88,8 @@ __device__ __forceinline__ RotState<T> rotate(T rho, T E, T p, T u, T v, T w, co
 // (SplitSurfaceFluxPI, src/equations/navierstokes/splitflux.f90:672-700), in the side frame.
 template <class T>
 __device__ __forceinline__ void pi_surface_flux(const RotState<T>& L, const RotState<T>& R, T* F) {
-  const T HL = (L.E + L.p) / L.rho;
-  const T HR = (R.E + R.p) / R.rho;
+  const T HL = (L.E + L.p) * L.srho;
+  const T HR = (R.E + R.p) * R.srho;
   const T rs = L.rho + R.rho;
   const T us = L.vn + R.vn;
   F[0] = T(0.25) * rs * us;
@@ -152,8 +153,9 @@ __device__ __forceinline__ void riemann_rot(const GasT<T>& g, int solver, bool s
     d[5] = d[4] - (d[2] - rv * d[0]) * rv - (d[3] - rw * d[0]) * rw;
     al[2] = d[2] - rv * d[0];
     al[3] = d[3] - rw * d[0];
-    al[1] = g.kappaM1 / (c * c) * (d[0] * (H - ru * ru) - d[5] + ru * d[1]);
-    al[0] = T(0.5) / c * (d[0] * (ru + c) - d[1] - c * al[1]);
+    const T sc = T(1) / c;
+    al[1] = g.kappaM1 * (sc * sc) * (d[0] * (H - ru * ru) - d[5] + ru * d[1]);
+    al[0] = T(0.5) * sc * (d[0] * (ru + c) - d[1] - c * al[1]);
     al[4] = d[0] - al[0] - al[1];
     for (int k = 0; k < 5; ++k) a[k] = fabs(a[k]);
   } else {  // RoeEntropyFix
@@ -161,7 +163,8 @@ __device__ __forceinline__ void riemann_rot(const GasT<T>& g, int solver, bool s
     const T cR = sqrt(g.kappa * R.p * R.srho);
     const T rd = sqrt(L.rho * R.rho);
     const T d1 = R.rho - L.rho, du = R.vn - L.vn, dv = R.vt1 - L.vt1, dw = R.vt2 - L.vt2, dp = R.p - L.p;
-    const T tmp = T(0.5) / (c * c);
+    const T sc = T(1) / c;
+    const T tmp = T(0.5) * (sc * sc);
     al[0] = tmp * (dp - rd * c * du);
     al[1] = d1 - dp * T(2) * tmp;
     al[2] = rd * dv;
