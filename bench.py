@@ -52,7 +52,7 @@ MEASURED_BYTES_PER_DOF = {"dg_fused": 364.1, "visc_faces": 143.3, "faces": 64.0}
 # FP64 floating-point operations per DOF of one launch (mean over the 5 stages of a step): rocprofv3
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 per dispatch, wave instructions x 64 lanes (every wave of
 # these kernels runs with all 64 lanes or skips a phase as a whole), FMA counted as 2; same summary
-MEASURED_FLOP_PER_DOF = {"dg_fused": 2219.6, "visc_faces": 413.7}
+MEASURED_FLOP_PER_DOF = {"dg_fused": 2048.6, "visc_faces": 391.0}
 FP64_PEAK_TFLOPS = 78.6
 RHS_EVALS = 1000
 
