@@ -416,9 +416,12 @@ __attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) 
     const Prim po = cons_to_prim(Uo, gas), pn = cons_to_prim(Un, gas);
     const RotState<double> So = rotate<double>(po.rho, Uo[4], po.p, po.u, po.v, po.w, nv, t1, t2);
     const RotState<double> Sn = rotate<double>(pn.rho, Un[4], pn.p, pn.u, pn.v, pn.w, nv, t1, t2);
-    RotAdj ao, an;
-    if (master) riemann_rot_vjp<SOLVER>(gas, So, Sn, lFr, ao, an);
-    else riemann_rot_vjp<SOLVER>(gas, Sn, So, lFr, an, ao);
+    // the master's state on the left, operands selected rather than branched on (one copy of the
+    // reverse; a wave may hold master and slave points)
+    const RotState<double> SL = master ? So : Sn, SR = master ? Sn : So;
+    RotAdj aL, aR;
+    riemann_rot_vjp<SOLVER>(gas, SL, SR, lFr, aL, aR);
+    const RotAdj ao = master ? aL : aR, an = master ? aR : aL;
     double gOwn[5], gNb[5];
     rot_state_adjoint(Uo, gas, nv, t1, t2, ao, gOwn);
     rot_state_adjoint(Un, gas, nv, t1, t2, an, gNb);

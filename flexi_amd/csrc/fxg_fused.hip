@@ -166,8 +166,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
       }
     }
     if (!wall) {
-      if (master) riemann_rot<RF>(gt, SOLVER, true, So, Sn, F);
-      else riemann_rot<RF>(gt, SOLVER, true, Sn, So, F);
+      // the master's state on the left, by selecting the operands rather than branching: one copy
+      // of the solver serves both sides, and a wave that holds master and slave points (N < 7)
+      // runs it once
+      const RotState<RF> SL = master ? So : Sn, SR = master ? Sn : So;
+      riemann_rot<RF>(gt, SOLVER, true, SL, SR, F);
     }
     const double lh = minus_side(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N];
     sSurf[x] = ms * (double)F[0] * se * lh;
