@@ -304,28 +304,93 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
     }
     __syncthreads();
   }
-  // Ut = -Ut*sJ and the register update; with st.lamSum also CALCTIMESTEP's per-element spectral
+  // Ut = -Ut*sJ, the register update and the new state's face values for the next stage; in the
+  // last stage of a step (LAM, always an LSERK stage) also CALCTIMESTEP's per-element spectral
   // radii of the new state (the time-step kernel's arithmetic, dt_node_lambda)
+  constexpr bool LAM_FIRST = LAM && STAGE;
   double un[5] = {0, 0, 0, 0, 0};
-  if (act) {
-    sjd = m.sJ[gid];
-    double ut[5];
+  if constexpr (!LAM_FIRST) {
+    if (act) {
+      sjd = m.sJ[gid];
+      double ut[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) ut[v] = (-(double)acc[v]) * sjd;
-    const size_t o = 5 * gid;
-    if (st.mode == 0) {
+      for (int v = 0; v < 5; ++v) ut[v] = (-(double)acc[v]) * sjd;
+      const size_t o = 5 * gid;
+      if (st.mode == 0) {
 #pragma unroll
-      for (int v = 0; v < 5; ++v) Ut[o + v] = ut[v];
-    } else {
+        for (int v = 0; v < 5; ++v) Ut[o + v] = ut[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          const double r = (st.mode == 1) ? ut[v] : rpre[v] * st.minusA + ut[v];
+          st.R[o + v] = r;
+          un[v] = upre[v] + r * st.b_dt;
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) st.U[o + v] = un[v];
+#pragma unroll 1
+        for (int ls = 1; ls <= 6; ++ls) {
+          if (!(sides & (1 << (ls - 1)))) continue;
+          double* dst = FUnext + 5 * (size_t)sIdx[face_point<N>(ls, i, j, k)];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) dst[v] = un[v];
+        }
+      }
+    }
+    if constexpr (LAM) {  // FP32-bulk: the metric terms from memory (not staged in full)
+      double lam[6] = {0, 0, 0, 0, 0, 0};
+      int isnan_rho = 0;
+      if (act) dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
+      double mx[6];
+      int nanr = 0;
+      double* sRd = reinterpret_cast<double*>(sF);  // the direction loop's last barrier freed it
+      block_lambda_max<NT>(lam, isnan_rho, sRd, reinterpret_cast<int*>(sRd + 6 * (NT / 64)), t, mx, nanr);
+      if (t == 0) {
+        st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
+        st.lamSum[2 * e + 1] = mx[3] + mx[4] + mx[5];
+        st.lamNan[e] = nanr;
+      }
+    }
+  } else {
+    // FP64: the spectral radii come before the global stores. The compiler puts a wait for every
+    // outstanding vector-memory operation of the wave before the reduction's cross-lane moves,
+    // which after the stores took the time of their write-back (about 300 us per launch at 32^3
+    // elements, N=7). The reduction goes through LDS rows, so that the new state and register stay
+    // in registers beside it within 128 VGPRs. (In FP32-bulk, held to 64 VGPRs, this order spills
+    // and is slower than the wait.)
+    double rn[5] = {0, 0, 0, 0, 0};
+    if (act) {
+      sjd = m.sJ[gid];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double r = (st.mode == 1) ? ut[v] : rpre[v] * st.minusA + ut[v];
-        st.R[o + v] = r;
-        un[v] = upre[v] + r * st.b_dt;
+        const double ut = (-(double)acc[v]) * sjd;
+        rn[v] = (st.mode == 1) ? ut : rpre[v] * st.minusA + ut;
+        un[v] = upre[v] + rn[v] * st.b_dt;
       }
+    }
+    double lam[6] = {0, 0, 0, 0, 0, 0};
+    int isnan_rho = 0;
+    if (act) {
+      double Jk[9];  // Ja^1..3 of the node as staged (the values the time-step kernel reads)
+#pragma unroll
+      for (int r = 0; r < 9; ++r) Jk[r] = (double)sJa[r * np + t];
+      dt_node_lambda(un, gas, sjd, Jk, Jk + 3, Jk + 6, lam, isnan_rho);
+    }
+    double mx[6];
+    int nanr = 0;
+    double* sRd = reinterpret_cast<double*>(sL);  // rows 0..5, freed by the direction loop
+    block_lambda_max_rows<NT, np>(lam, isnan_rho, sRd, reinterpret_cast<int*>(sRd + 6 * np + 6), t, mx, nanr);
+    if (t == 0) {
+      st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
+      st.lamSum[2 * e + 1] = mx[3] + mx[4] + mx[5];
+      st.lamNan[e] = nanr;
+    }
+    if (act) {
+      const size_t o = 5 * gid;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) st.R[o + v] = rn[v];
 #pragma unroll
       for (int v = 0; v < 5; ++v) st.U[o + v] = un[v];
-      // face values of the new state for the next stage
 #pragma unroll 1
       for (int ls = 1; ls <= 6; ++ls) {
         if (!(sides & (1 << (ls - 1)))) continue;
@@ -333,29 +398,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
 #pragma unroll
         for (int v = 0; v < 5; ++v) dst[v] = un[v];
       }
-    }
-  }
-  if constexpr (LAM) {  // the direction loop's last barrier freed the viscous-flux rows
-    double lam[6] = {0, 0, 0, 0, 0, 0};
-    int isnan_rho = 0;
-    if (act) {
-      if constexpr (sizeof(RF) == 8) {
-        double Jk[9];  // Ja^1..3 of the node as staged (FP64: the values the time-step kernel reads)
-#pragma unroll
-        for (int r = 0; r < 9; ++r) Jk[r] = (double)sJa[r * np + t];
-        dt_node_lambda(un, gas, sjd, Jk, Jk + 3, Jk + 6, lam, isnan_rho);
-      } else {
-        dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
-      }
-    }
-    double mx[6];
-    int nanr = 0;
-    double* sRd = reinterpret_cast<double*>(sF);
-    block_lambda_max<NT>(lam, isnan_rho, sRd, reinterpret_cast<int*>(sRd + 6 * (NT / 64)), t, mx, nanr);
-    if (t == 0) {
-      st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
-      st.lamSum[2 * e + 1] = mx[3] + mx[4] + mx[5];
-      st.lamNan[e] = nanr;
     }
   }
 }

@@ -247,4 +247,44 @@ __device__ __forceinline__ void block_lambda_max(double* lam, int isnan_rho, dou
   }
 }
 
+// The same maxima through LDS rows: every node writes its six values, then the waves of the block
+// reduce one value each (64 lanes over the nodes, then a cross-lane tree; a wave takes several
+// values when the block has fewer than six). The values leave the registers at once instead of
+// being carried through six interleaved cross-lane trees. red needs 6*NP + 6 doubles and nanflag
+// NT/64 ints of LDS; every thread must call it with lam[] set for t < NP; the result is valid in
+// thread 0. Two barriers.
+template <int NT, int NP>
+__device__ __forceinline__ void block_lambda_max_rows(const double* lam, int isnan_rho, double* red, int* nanflag,
+                                                      int t, double* mx, int& nanr) {
+  if (t < NP) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) red[c * NP + t] = lam[c];
+  }
+  const unsigned long long any = __ballot(isnan_rho);
+  if ((t & 63) == 0) nanflag[t >> 6] = any ? 1 : 0;
+  __syncthreads();
+  double* part = red + 6 * NP;
+  const int l = t & 63;
+  for (int c = t >> 6; c < 6; c += NT / 64) {
+    // the lanes t >= NP of block_lambda_max contribute zeros; so does the start value here
+    double x = NP < NT ? 0.0 : -__builtin_inf();
+    for (int p = l; p < NP; p += 64) {
+      const double y = red[c * NP + p];
+      x = (y > x || y != y) ? y : x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double y = __shfl_down(x, off, 64);
+      x = (y > x || y != y) ? y : x;
+    }
+    if (l == 0) part[c] = x;
+  }
+  __syncthreads();
+  if (t == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mx[c] = part[c];
+    nanr = 0;
+    for (int w = 0; w < NT / 64; ++w) nanr |= nanflag[w];
+  }
+}
+
 }  // namespace fxg
