@@ -27,18 +27,55 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : (N == 7 
   __shared__ double sB[3 * n * n + 4 * n];
   const int e = blockIdx.x, t = threadIdx.x;
   const double* Ue = U + (size_t)5 * np * e;
-  if (raw)
-    for (int x = t; x < 5 * np; x += NT) sRaw[x] = Ue[x];
-  else if (t < np) {
-    const Prim q = cons_to_prim(Ue + 5 * t, gas);
+  constexpr int NB = Bnd<N>::NB;         // boundary nodes
+  constexpr int NBAS = 3 * n * n + 4 * n;
+  static_assert(NBAS <= NT, "one basis entry per lane");
+  constexpr int RS = (NF + NT - 1) / NT;  // own face points per lane
+  constexpr int RU = raw ? (5 * np + NT - 1) / NT : 5;
+  // Every global load of the staging is issued before the first wait, instead of one wait per
+  // loop of loads: the side slots first (the neighbour states depend on them), then U (FP64: the
+  // element's values with contiguous lanes; FP32-bulk: the lane's node) and the basis.
+  unsigned own[RS];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int x = t + r * NT;
+    own[r] = x < NF ? face_slot_tab<N>(m, e, x) : 0u;
+  }
+  double ul[RU];
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    if constexpr (raw) {
+      const int x = t + r * NT;
+      ul[r] = x < 5 * np ? Ue[x] : 0.0;
+    } else {
+      ul[r] = t < np ? Ue[5 * t + r] : 1.0;
+    }
+  }
+  const double bas = t < NBAS ? m.basis[t] : 0.0;
+  double cn[RS][5];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const double* c = FU + 5 * (size_t)slot_nbr<N>(own[r]);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) cn[r][v] = t + r * NT < NF ? c[v] : 1.0;
+  }
+  if constexpr (raw) {
+#pragma unroll
+    for (int r = 0; r < RU; ++r)
+      if (t + r * NT < 5 * np) sRaw[t + r * NT] = ul[r];
+  } else if (t < np) {
+    const Prim q = cons_to_prim(ul, gas);
     sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
   }
-  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
-  for (int x = t; x < NF; x += NT) {
-    const unsigned own = face_slot_tab<N>(m, e, x);
-    sIdx[x] = own;
-    const Prim q = cons_to_prim(FU + 5 * (size_t)slot_nbr<N>(own), gas);
-    sNb[x] = q.u; sNb[NF + x] = q.v; sNb[2 * NF + x] = q.w; sNb[3 * NF + x] = q.T;
+  if (t < NBAS) sB[t] = bas;
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int x = t + r * NT;
+    if (x < NF) {
+      sIdx[x] = own[r];
+      const Prim q = cons_to_prim(cn[r], gas);
+      sNb[x] = q.u; sNb[NF + x] = q.v; sNb[2 * NF + x] = q.w; sNb[3 * NF + x] = q.T;
+    }
   }
   __syncthreads();
   if (raw) {
@@ -50,7 +87,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : (N == 7 
   }
   // one lane per boundary node, in boundary-rank order (the Jsoa rows): lanes 0..NB-1, so the
   // waves beyond NB/64 end here instead of idling through the lifting
-  constexpr int NB = Bnd<N>::NB;
   if (t >= NB) return;
   int i, j, k;
   bnd_node<N>(t, i, j, k);
