@@ -323,21 +323,44 @@ __attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) 
   const bool act = t < np;
   const size_t gid = (size_t)np * e + (act ? t : 0);
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
-  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
-  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot<N>(m, e, x);
+  // Staging. The independent loads (ElemToSide of the lane's first own face point, the basis, the
+  // node's U, sJ, lam, ub and out) are issued together, before the first wait; all stores come
+  // after the loads (a store between two loads makes the second wait for the first's write-back).
+  constexpr int NBAS = 3 * n * n + 4 * n;
+  static_assert(NBAS <= NT, "one basis entry per lane");
+  const int2 sfe = face_side_entry<N>(m, e, t < NF ? t : 0);  // (lanes t >= NF do not use it)
+  const double bas = t < NBAS ? m.basis[t] : 0.0;
+  double c5[5] = {1.0, 1.0, 1.0, 1.0, 1.0}, l0[5] = {0, 0, 0, 0, 0}, u0[5] = {0, 0, 0, 0, 0},
+         o5[5] = {0, 0, 0, 0, 0}, sj = 0.0;
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) c5[v] = U[5 * gid + v];
+    sj = m.sJ[gid];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) l0[v] = lam[5 * gid + v];
+    if (ub) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) u0[v] = ub[5 * gid + v];
+    }
+    if (pd.XU) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) o5[v] = out[5 * gid + v];
+    }
+  }
+  if (t < NBAS) sB[t] = bas;
+  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot_from<N>(m, x == t ? sfe : face_side_entry<N>(m, e, x), x);
   if (pd.XU) __syncthreads();  // uniform over the launch
   if (act) {
-    const Prim q = cons_to_prim(U + 5 * gid, gas);
-    sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (U[5 * gid + 4] + q.p) / q.rho;
+    const Prim q = cons_to_prim(c5, gas);
+    sA[t] = q.rho; sA[np + t] = q.p; sA[2 * np + t] = (c5[4] + q.p) / q.rho;
     sA[3 * np + t] = q.u; sA[4 * np + t] = q.v; sA[5 * np + t] = q.w;
-    const double sj = m.sJ[gid];
     double l5[5];
     if (pd.XU) {
       // the register update of the previous reverse stage: ub += its out + the neighbours' face
       // parts, rbar = a*(rbar + cub*ub) (rbar lives in lam)
       double g5[5];
 #pragma unroll
-      for (int v = 0; v < 5; ++v) g5[v] = out[5 * gid + v];
+      for (int v = 0; v < 5; ++v) g5[v] = o5[v];
       const int sides = node_sides<N>(i, j, k);
 #pragma unroll 1
       for (int ls = 1; ls <= 6; ++ls) {
@@ -348,16 +371,15 @@ __attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) 
       }
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double u0 = ub[5 * gid + v];
-        const double r = pd.a * (lam[5 * gid + v] + pd.cub * u0);
-        const double u1 = u0 + g5[v];
+        const double r = pd.a * (l0[v] + pd.cub * u0[v]);
+        const double u1 = u0[v] + g5[v];
         lam[5 * gid + v] = r;
         ub[5 * gid + v] = u1;
         l5[v] = r + cub * u1;
       }
     } else {
 #pragma unroll
-      for (int v = 0; v < 5; ++v) l5[v] = lam[5 * gid + v] + (ub ? cub * ub[5 * gid + v] : 0.0);
+      for (int v = 0; v < 5; ++v) l5[v] = l0[v] + (ub ? cub * u0[v] : 0.0);
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) sMu[v * np + t] = -sj * l5[v];
@@ -520,18 +542,34 @@ __global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) v
   const int e = blockIdx.x, t = threadIdx.x;
   const bool act = t < np;
   const size_t gid = (size_t)np * e + (act ? t : 0);
-  for (int x = t; x < 3 * n * n + 4 * n; x += NT) sB[x] = m.basis[x];
+  // Staging: the independent loads (ElemToSide of the lane's first own face point, the basis, the
+  // node's U, sJ, lam and ub) are issued together, before the first wait.
+  constexpr int NBAS = 3 * n * n + 4 * n;
+  static_assert(NBAS <= NT, "one basis entry per lane");
+  const int2 sfe = face_side_entry<N>(m, e, t < NF ? t : 0);  // (lanes t >= NF do not use it)
+  const double bas = t < NBAS ? m.basis[t] : 0.0;
+  double c5[5] = {1.0, 1.0, 1.0, 1.0, 1.0}, l4[4] = {0, 0, 0, 0}, u4[4] = {0, 0, 0, 0}, sjn = 0.0;
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) c5[v] = U[5 * gid + v];
+    sjn = m.sJ[gid];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) l4[c] = lam[5 * gid + 1 + c];
+    if (ub) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) u4[c] = ub[5 * gid + 1 + c];
+    }
+  }
+  if (t < NBAS) sB[t] = bas;
   Prim q{};
   if (act) {
-    q = cons_to_prim(U + 5 * gid, gas);
+    q = cons_to_prim(c5, gas);
     sP[t] = q.u; sP[np + t] = q.v; sP[2 * np + t] = q.w; sP[3 * np + t] = q.T;
-    const double sj = m.sJ[gid];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      sMu[c * np + t] = -sj * (lam[5 * gid + 1 + c] + (ub ? cub * ub[5 * gid + 1 + c] : 0.0));
+    for (int c = 0; c < 4; ++c) sMu[c * np + t] = -sjn * (l4[c] + (ub ? cub * u4[c] : 0.0));
   }
   for (int x = t; x < NF; x += NT) {
-    const unsigned own = face_slot<N>(m, e, x);
+    const unsigned own = face_slot_from<N>(m, x == t ? sfe : face_side_entry<N>(m, e, x), x);
     sIdx[x] = own;
     const Prim qn = cons_to_prim(FUa + 5 * (size_t)slot_nbr<N>(own), gas);
     sNb[x] = qn.u; sNb[NF + x] = qn.v; sNb[2 * NF + x] = qn.w; sNb[3 * NF + x] = qn.T; sNb[4 * NF + x] = qn.rho;

@@ -53,14 +53,24 @@ __device__ __forceinline__ int face_point(int ls, int i, int j, int k) {
 }
 
 // Side-array slot of own face point x of element e: ((side*2 + slot)*nf + pq), slot 0 master.
+// In two steps for callers that issue other loads between them: the ElemToSide entry (side, flip)
+// of the point's local side, then the slot through the inverse S2V2 map.
 template <int N>
-__device__ __forceinline__ unsigned face_slot(const DevMesh& m, int e, int x) {
+__device__ __forceinline__ int2 face_side_entry(const DevMesh& m, int e, int x) {
+  constexpr int nf = (N + 1) * (N + 1);
+  const int ls = x / nf + 1;
+  return make_int2(m.E2S[2 * (ls - 1) + 12 * e] - 1, m.E2S[2 * (ls - 1) + 1 + 12 * e]);
+}
+template <int N>
+__device__ __forceinline__ unsigned face_slot_from(const DevMesh& m, int2 sf, int x) {
   constexpr int n = N + 1, nf = n * n;
   const int ls = x / nf + 1, ab = x - (ls - 1) * nf;
-  const int side = m.E2S[2 * (ls - 1) + 12 * e] - 1;
-  const int flip = m.E2S[2 * (ls - 1) + 1 + 12 * e];
-  const int* pq = m.v2s2 + 2 * (ab + nf * (flip + 5 * (ls - 1)));
-  return (unsigned)(((size_t)side * 2 + (flip ? 1 : 0)) * nf + pq[0] + n * pq[1]);
+  const int* pq = m.v2s2 + 2 * (ab + nf * (sf.y + 5 * (ls - 1)));
+  return (unsigned)(((size_t)sf.x * 2 + (sf.y ? 1 : 0)) * nf + pq[0] + n * pq[1]);
+}
+template <int N>
+__device__ __forceinline__ unsigned face_slot(const DevMesh& m, int e, int x) {
+  return face_slot_from<N>(m, face_side_entry<N>(m, e, x), x);
 }
 // The same from the table of the fused path (m.slotTab, filled by launch_slot_table): one load
 // instead of ElemToSide followed by the inverse S2V2 map.
