@@ -348,7 +348,15 @@ __attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) 
     }
   }
   if (t < NBAS) sB[t] = bas;
-  for (int x = t; x < NF; x += NT) sIdx[x] = face_slot_from<N>(m, x == t ? sfe : face_side_entry<N>(m, e, x), x);
+  for (int x = t; x < NF; x += NT) {
+    const unsigned own = face_slot_from<N>(m, x == t ? sfe : face_side_entry<N>(m, e, x), x);
+    sIdx[x] = own;
+    if (pd.XU) {  // the neighbours' face parts, staged in sLf (free until the surface phase)
+      const size_t nb = slot_nbr<N>(own);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sLf[v * NF + x] = pd.XU[5 * nb + v];
+    }
+  }
   if (pd.XU) __syncthreads();  // uniform over the launch
   if (act) {
     const Prim q = cons_to_prim(c5, gas);
@@ -365,9 +373,9 @@ __attribute__((amdgpu_waves_per_eu(SOLVER == 3 && (N == 5 || N == 6) ? 3 : 1))) 
 #pragma unroll 1
       for (int ls = 1; ls <= 6; ++ls) {
         if (!(sides & (1 << (ls - 1)))) continue;
-        const unsigned nb = slot_nbr<N>(sIdx[face_point<N>(ls, i, j, k)]);
+        const int x = face_point<N>(ls, i, j, k);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) g5[v] += pd.XU[5 * (size_t)nb + v];
+        for (int v = 0; v < 5; ++v) g5[v] += sLf[v * NF + x];
       }
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -568,10 +576,20 @@ __global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
     for (int c = 0; c < 4; ++c) sMu[c * np + t] = -sjn * (l4[c] + (ub ? cub * u4[c] : 0.0));
   }
+  // where rows 8..11 of sW (free until the lifting adjoint is formed) hold NF values, the
+  // neighbours' viscous shares Xg are staged there with the face states, so that the side loop
+  // below reads them from LDS instead of waiting for one load per side
+  constexpr bool XG_LDS = np >= NF;  // N >= 5
+  double* sXg = sW + 8 * np;
   for (int x = t; x < NF; x += NT) {
     const unsigned own = face_slot_from<N>(m, x == t ? sfe : face_side_entry<N>(m, e, x), x);
     sIdx[x] = own;
-    const Prim qn = cons_to_prim(FUa + 5 * (size_t)slot_nbr<N>(own), gas);
+    const size_t nb = slot_nbr<N>(own);
+    if constexpr (XG_LDS) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sXg[c * NF + x] = Xg[4 * nb + c];
+    }
+    const Prim qn = cons_to_prim(FUa + 5 * nb, gas);
     sNb[x] = qn.u; sNb[NF + x] = qn.v; sNb[2 * NF + x] = qn.w; sNb[3 * NF + x] = qn.T; sNb[4 * NF + x] = qn.rho;
   }
   __syncthreads();
@@ -609,13 +627,13 @@ __global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) v
       const int dir = dir_of(ls);
       const double sg = normal_sign(ls);
       const double lh = minus_side(ls) ? sB[B.L_HatMinus()] : sB[B.L_HatPlus() + N];
-      const unsigned own = sIdx[face_point<N>(ls, i, j, k)];
-      const double* xg = Xg + 4 * (size_t)slot_nbr<N>(own);
+      const int x = face_point<N>(ls, i, j, k);
+      const double* xg = Xg + 4 * (size_t)slot_nbr<N>(sIdx[x]);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const double a = 0.5 * lh * sMu[c * np + t] + xg[c];
+        const double a = 0.5 * lh * sMu[c * np + t] + (XG_LDS ? sXg[c * NF + x] : xg[c]);
 #pragma unroll
-        for (int x = 0; x < 3; ++x) lf[x][c] += a * sg * Ja[3 * dir + x];
+        for (int r = 0; r < 3; ++r) lf[r][c] += a * sg * Ja[3 * dir + r];
       }
     }
     // diff_flux adjoint (flux.f90:149-241): stresses, velocities, temperature gradient
@@ -664,7 +682,6 @@ __global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) v
       const int dir = dir_of(ls);
       const double s = normal_sign(ls) * 0.5 * (minus_side(ls) ? lhm : lhp);
       const int x = face_point<N>(ls, i, j, k);
-      const unsigned own = sIdx[x];
       double z[4];
 #pragma unroll
       for (int v2 = 0; v2 < 4; ++v2) {
@@ -674,11 +691,19 @@ __global__ __launch_bounds__(D<N>::NT) __attribute__((amdgpu_waves_per_eu(4))) v
       const Prim qn{sNb[4 * NF + x], sNb[x], sNb[NF + x], sNb[2 * NF + x], 0.0, sNb[3 * NF + x]};
       double lP[5];
       prim_adjoint(qn, 0.0, gas, 0.0, z[0], z[1], z[2], 0.0, z[3], 0.0, lP);
+      // in place of the neighbour values of this face point (read above; no other lane has x)
 #pragma unroll
-      for (int v = 0; v < 5; ++v) XU[5 * (size_t)own + v] += lP[v];
+      for (int v = 0; v < 5; ++v) sNb[v * NF + x] = lP[v];
     }
   }
   __syncthreads();
+  // into the neighbour's exchange slots, one lane per own face point: the read-modify-writes of
+  // a node's sides then do not wait for each other's stores
+  for (int x = t; x < NF; x += NT) {
+    const unsigned own = sIdx[x];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) XU[5 * (size_t)own + v] += sNb[v * NF + x];
+  }
   if (!act) return;
   // transposed line derivatives: dq/dxi_d at node i = sum_l D_T(l,i) q_l
   const double* DT = sB + B.D_T();
