@@ -378,8 +378,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
     }
     double mx[6];
     int nanr = 0;
-    double* sRd = reinterpret_cast<double*>(sL);  // rows 0..5, freed by the direction loop
-    block_lambda_max_rows<NT, np>(lam, isnan_rho, sRd, reinterpret_cast<int*>(sRd + 6 * np + 6), t, mx, nanr);
+    // rows 0..5 and the viscous-flux rows, freed by the direction loop (not the metric rows, which
+    // other waves may still be reading above)
+    double* sRd = reinterpret_cast<double*>(sL);
+    double* sPart = reinterpret_cast<double*>(sF);
+    block_lambda_max_rows<NT, np>(lam, isnan_rho, sRd, sPart, reinterpret_cast<int*>(sPart + 6), t, mx, nanr);
     if (t == 0) {
       st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
       st.lamSum[2 * e + 1] = mx[3] + mx[4] + mx[5];

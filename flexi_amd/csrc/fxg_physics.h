@@ -250,12 +250,13 @@ __device__ __forceinline__ void block_lambda_max(double* lam, int isnan_rho, dou
 // The same maxima through LDS rows: every node writes its six values, then the waves of the block
 // reduce one value each (64 lanes over the nodes, then a cross-lane tree; a wave takes several
 // values when the block has fewer than six). The values leave the registers at once instead of
-// being carried through six interleaved cross-lane trees. red needs 6*NP + 6 doubles and nanflag
-// NT/64 ints of LDS; every thread must call it with lam[] set for t < NP; the result is valid in
-// thread 0. Two barriers.
+// being carried through six interleaved cross-lane trees. red needs 6*NP doubles, part 6 doubles
+// and nanflag NT/64 ints of LDS (nanflag is written before the first barrier: it must not overlap
+// anything another wave may still read); every thread must call it with lam[] set for t < NP; the
+// result is valid in thread 0. Two barriers.
 template <int NT, int NP>
-__device__ __forceinline__ void block_lambda_max_rows(const double* lam, int isnan_rho, double* red, int* nanflag,
-                                                      int t, double* mx, int& nanr) {
+__device__ __forceinline__ void block_lambda_max_rows(const double* lam, int isnan_rho, double* red, double* part,
+                                                      int* nanflag, int t, double* mx, int& nanr) {
   if (t < NP) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) red[c * NP + t] = lam[c];
@@ -263,7 +264,6 @@ __device__ __forceinline__ void block_lambda_max_rows(const double* lam, int isn
   const unsigned long long any = __ballot(isnan_rho);
   if ((t & 63) == 0) nanflag[t >> 6] = any ? 1 : 0;
   __syncthreads();
-  double* part = red + 6 * NP;
   const int l = t & 63;
   for (int c = t >> 6; c < 6; c += NT / 64) {
     // the lanes t >= NP of block_lambda_max contribute zeros; so does the start value here
