@@ -246,10 +246,13 @@ def test_fused_steps_reuse_face_values_between_calls():
     assert rel(gpu.get_state() - U2, U2o - U2) <= 1e-11
 
 
-def test_fused_last_stage_time_step_equals_time_step_kernel():
-    # after a fused step, CALCTIMESTEP uses the spectral radii the last stage stored; a fresh
-    # set_state forces the time-step kernel on the same state
-    mesh, m, oracle, gpu, U = setup(7, orc.NODE_GL, True, orc.RIEMANN_ROE, shape=(3, 2, 2))
+@pytest.mark.parametrize("precision", ["fp64", "fp32-bulk"])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6, 7])
+def test_fused_last_stage_time_step_equals_time_step_kernel(N, precision):
+    # after a fused step, CALCTIMESTEP uses the spectral radii the last stage stored (its block
+    # reduction takes 1, 2, 4, 6 or 8 waves over N); a fresh set_state forces the time-step kernel
+    # on the same state
+    mesh, m, oracle, gpu, U = setup(N, orc.NODE_GL, True, orc.RIEMANN_ROE, shape=(3, 2, 2), precision=precision)
     gpu.set_state(U)
     for it in range(2):
         dt, err = gpu.CalcTimeStep()
@@ -261,6 +264,15 @@ def test_fused_last_stage_time_step_equals_time_step_kernel():
     assert err1 == err2 == 0
     assert abs(dt1 - dt2) <= 1e-15 * dt2
     np.testing.assert_allclose(e1, e2, rtol=1e-15, atol=0)
+    # a NaN density carried into the last stage's new state is reported as by the time-step
+    # kernel (errType 1, density first)
+    Un = np.asfortranarray(gpu.get_state())
+    Un[0, 1, 1, 1, 5] = np.nan
+    gpu.set_state(Un)
+    gpu.dt = dt1
+    gpu.TimeStepByLSERKW2(0.0)
+    dt3, err3 = gpu.CalcTimeStep()
+    assert err3 == 1
 
 
 @pytest.mark.parametrize("scheme", ["niegemannrk4-14", "toulorgerk4-8c", "toulorgerk3-7c", "toulorgerk4-8f",
