@@ -57,13 +57,14 @@ struct LdsB {
 // The Riemann solver is a template parameter: each instance then carries only its own solver's
 // code, which keeps the kernel within 128 VGPRs (two workgroups per CU at N=7) without spills.
 // In FP32-bulk the LDS arrays are FP32 (38 KB at N=7) and the kernel is held to 64 VGPRs, so
-// four workgroups share a CU (six waves per SIMD below N=7, where 8 would spill).
+// four workgroups share a CU (six waves per SIMD below N=7, where 8 would spill, and in the last
+// stage of a step).
 // LAM: the last stage of a step, which also forms CALCTIMESTEP's per-element spectral radii of the
 // new state; in FP64 from the metric terms staged in LDS (the same values as the time-step kernel
 // reads), so they are not read from memory a second time.
 template <int N, class RF, int SOLVER, bool BC, bool LAM>
 __global__ __launch_bounds__(D<N>::NT)
-__attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
+__attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 && !LAM ? 8 : 6) : 1))) void k_dg_fused(const double* U, DevMesh m, const double* __restrict__ FU,
                                                      double* __restrict__ FUnext, const RF* __restrict__ FG,
                                                      Gas gas, double* Ut, RKStage st, RF* __restrict__ Gout,
                                                      const double* __restrict__ bcP) {
@@ -307,9 +308,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
   // Ut = -Ut*sJ, the register update and the new state's face values for the next stage; in the
   // last stage of a step (LAM, always an LSERK stage) also CALCTIMESTEP's per-element spectral
   // radii of the new state (the time-step kernel's arithmetic, dt_node_lambda)
-  constexpr bool LAM_FIRST = LAM && STAGE;
   double un[5] = {0, 0, 0, 0, 0};
-  if constexpr (!LAM_FIRST) {
+  if constexpr (!LAM) {
     if (act) {
       sjd = m.sJ[gid];
       double ut[5];
@@ -337,27 +337,14 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
         }
       }
     }
-    if constexpr (LAM) {  // FP32-bulk: the metric terms from memory (not staged in full)
-      double lam[6] = {0, 0, 0, 0, 0, 0};
-      int isnan_rho = 0;
-      if (act) dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
-      double mx[6];
-      int nanr = 0;
-      double* sRd = reinterpret_cast<double*>(sF);  // the direction loop's last barrier freed it
-      block_lambda_max<NT>(lam, isnan_rho, sRd, reinterpret_cast<int*>(sRd + 6 * (NT / 64)), t, mx, nanr);
-      if (t == 0) {
-        st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
-        st.lamSum[2 * e + 1] = mx[3] + mx[4] + mx[5];
-        st.lamNan[e] = nanr;
-      }
-    }
   } else {
-    // FP64: the spectral radii come before the global stores. The compiler puts a wait for every
-    // outstanding vector-memory operation of the wave before the reduction's cross-lane moves,
-    // which after the stores took the time of their write-back (about 300 us per launch at 32^3
-    // elements, N=7). The reduction goes through LDS rows, so that the new state and register stay
-    // in registers beside it within 128 VGPRs. (In FP32-bulk, held to 64 VGPRs, this order spills
-    // and is slower than the wait.)
+    // The spectral radii come before the global stores. The compiler puts a wait for every
+    // outstanding vector-memory operation of the wave before the reduction's cross-lane moves (and
+    // before the metric loads' use in FP32-bulk), which after the stores took the time of their
+    // write-back (about 300 us per launch at 32^3 elements, N=7). The reduction goes through LDS
+    // rows, so that the new state and register stay in registers beside it: within 128 VGPRs in
+    // FP64; in FP32-bulk this instance is held to six waves per SIMD (76 VGPRs) instead of eight,
+    // which spilled.
     double rn[5] = {0, 0, 0, 0, 0};
     if (act) {
       sjd = m.sJ[gid];
@@ -371,17 +358,22 @@ __attribute__((amdgpu_waves_per_eu(sizeof(RF) == 4 ? (N == 7 ? 8 : 6) : 1))) voi
     double lam[6] = {0, 0, 0, 0, 0, 0};
     int isnan_rho = 0;
     if (act) {
-      double Jk[9];  // Ja^1..3 of the node as staged (the values the time-step kernel reads)
+      if constexpr (STAGE) {
+        double Jk[9];  // Ja^1..3 of the node as staged (the values the time-step kernel reads)
 #pragma unroll
-      for (int r = 0; r < 9; ++r) Jk[r] = (double)sJa[r * np + t];
-      dt_node_lambda(un, gas, sjd, Jk, Jk + 3, Jk + 6, lam, isnan_rho);
+        for (int r = 0; r < 9; ++r) Jk[r] = (double)sJa[r * np + t];
+        dt_node_lambda(un, gas, sjd, Jk, Jk + 3, Jk + 6, lam, isnan_rho);
+      } else {  // FP32-bulk: the metric terms from memory (not staged in full)
+        dt_node_lambda(un, gas, sjd, m.Mf + 3 * gid, m.Mg + 3 * gid, m.Mh + 3 * gid, lam, isnan_rho);
+      }
     }
     double mx[6];
     int nanr = 0;
-    // rows 0..5 and the viscous-flux rows, freed by the direction loop (not the metric rows, which
-    // other waves may still be reading above)
+    // LDS freed by the direction loop: the rows of 6 NP doubles from the start, the partial maxima
+    // and flags in the viscous-flux rows (FP64; not in the metric rows, which other waves may still
+    // be reading above) or right after them (FP32-bulk, whose metric terms come from memory)
     double* sRd = reinterpret_cast<double*>(sL);
-    double* sPart = reinterpret_cast<double*>(sF);
+    double* sPart = STAGE ? reinterpret_cast<double*>(sF) : sRd + 6 * np;
     block_lambda_max_rows<NT, np>(lam, isnan_rho, sRd, sPart, reinterpret_cast<int*>(sPart + 6), t, mx, nanr);
     if (t == 0) {
       st.lamSum[2 * e] = mx[0] + mx[1] + mx[2];
