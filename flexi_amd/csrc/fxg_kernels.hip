@@ -49,6 +49,24 @@ __device__ __forceinline__ void load_basis(const double* __restrict__ g, double*
   for (int x = t; x < 3 * n * n + 4 * n; x += nt) s[x] = g[x];
 }
 
+// The element's U (5*np doubles, contiguous lanes) and the basis into LDS, every load issued
+// before the first wait: a loop with the wait inside would spend one memory latency per
+// iteration (five at N=7).
+template <int N, int NT>
+__device__ __forceinline__ void stage_u_basis(const double* __restrict__ Ue, double* sU, const double* __restrict__ g,
+                                              double* sB, int t) {
+  constexpr int n = N + 1, np = n * n * n, NBAS = 3 * n * n + 4 * n, RU = (5 * np + NT - 1) / NT;
+  static_assert(NBAS <= NT, "one basis entry per lane");
+  double ul[RU];
+#pragma unroll
+  for (int r = 0; r < RU; ++r) ul[r] = t + r * NT < 5 * np ? Ue[t + r * NT] : 0.0;
+  const double b = t < NBAS ? g[t] : 0.0;
+#pragma unroll
+  for (int r = 0; r < RU; ++r)
+    if (t + r * NT < 5 * np) sU[t + r * NT] = ul[r];
+  if (t < NBAS) sB[t] = b;
+}
+
 // ---------------------------------------------------------------------------------------------
 // ProlongToFace of the conservative state (prolongtoface.t90:21-144): every element writes the
 // face values of its six sides into the master or slave face array of the side, reordered into
@@ -62,8 +80,7 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_prolong_u(const double* __restri
   const int e = blockIdx.x;
   const int t = threadIdx.x;
   const double* Ue = U + (size_t)5 * np * e;
-  for (int x = t; x < 5 * np; x += NT) sU[x] = Ue[x];
-  load_basis<N>(m.basis, sB, t, NT);
+  stage_u_basis<N, NT>(Ue, sU, m.basis, sB, t);
   __syncthreads();
   const BasisLayout B{n};
   for (int x = t; x < 6 * nf; x += NT) {
@@ -109,8 +126,7 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_lift(const double* __restrict__ 
   const int e = blockIdx.x;
   const int t = threadIdx.x;
   const double* Ue = U + (size_t)5 * np * e;
-  for (int x = t; x < 5 * np; x += NT) sA[x] = Ue[x];
-  load_basis<N>(m.basis, sB, t, NT);
+  stage_u_basis<N, NT>(Ue, sA, m.basis, sB, t);
   __syncthreads();
   double* sP = sA + 5 * np;  // lifting variables (u,v,w,T), SoA
   if (t < np) {
@@ -403,8 +419,7 @@ __global__ __launch_bounds__(Dim<N>::NT) void k_volint(const double* __restrict_
   const int e = blockIdx.x;
   const int t = threadIdx.x;
   const double* Ue = U + (size_t)5 * np * e;
-  for (int x = t; x < 5 * np; x += NT) sA[x] = Ue[x];
-  load_basis<N>(m.basis, sB, t, NT);
+  stage_u_basis<N, NT>(Ue, sA, m.basis, sB, t);
   __syncthreads();
   const bool act = t < np;
   const int i = t % n, j = (t / n) % n, k = t / (n * n);
